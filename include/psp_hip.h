@@ -1,0 +1,177 @@
+/*
+ * psp_hip.h -- C ABI of the MI355X-native sweeping-preconditioned Helmholtz
+ * solver.  Plain pointers and sizes only; no C++ or torch types cross it.
+ *
+ * The reference (PSP) exposes a header-only C++ template API and no C ABI.
+ * Each entry point below names the reference method it replaces
+ * (paths relative to the PSP source tree).  The C++ mirror of that API,
+ * psp::DistHelmholtz<double> (psp_amd/include/psp/dist_helmholtz.hpp), is a
+ * thin layer over these functions.
+ *
+ * Conventions
+ *  - Complex data is interleaved (re, im) doubles, layout-compatible with
+ *    std::complex<double>.
+ *  - Right-hand-side blocks use the DistUniformGrid layout at one process
+ *    (include/psp/dist_uniform_grid.hpp:322-341): entry (x, y, z, r) at
+ *    ((x + y*nx + z*nx*ny) * k + r).
+ *  - Functions never throw; they return a status code (PSP_HIP_OK == 0) and
+ *    leave a message readable through psp_hip_last_error().
+ *  - The context owns all device memory.  Calls on one context are
+ *    serialized on the context's stream; use one context per host thread.
+ */
+#ifndef PSP_HIP_H
+#define PSP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+enum {
+    PSP_HIP_OK = 0,
+    PSP_HIP_ERR_INVALID = 1,   /* LogicError: invalid argument            */
+    PSP_HIP_ERR_SHALLOW = 2,   /* LogicError: domain too shallow (main.hpp:38-41) */
+    PSP_HIP_ERR_NAN = 3,       /* RuntimeError: NaN in norms (solve.hpp:280-282, 401-403) */
+    PSP_HIP_ERR_BREAKDOWN = 4, /* exact Arnoldi breakdown (solve.hpp:404-414) */
+    PSP_HIP_ERR_NOCONV = 5,    /* iteration cap reached (the reference has none) */
+    PSP_HIP_ERR_STATE = 6,     /* called before psp_hip_factor           */
+    PSP_HIP_ERR_HIP = 7,       /* HIP runtime error / no device          */
+    PSP_HIP_ERR_MEMORY = 8     /* device allocation failed               */
+};
+
+/* Boundary (include/psp/discretization.hpp:15). */
+enum { PSP_HIP_PML = 0, PSP_HIP_DIRICHLET = 1 };
+
+/* Front precision: FP64 fronts reproduce PSP's method; FP32 fronts are
+ * factored and applied in complex float inside the FP64 GMRES. */
+enum { PSP_HIP_FRONTS_FP64 = 0, PSP_HIP_FRONTS_FP32 = 1 };
+
+/* PanelScheme (include/psp/dist_helmholtz.hpp:15-19). */
+enum { PSP_HIP_LDL_1D = 0, PSP_HIP_LDL_SELINV_1D = 1 };
+
+/* Flags for psp_hip_solve. */
+enum {
+    PSP_HIP_DEVICE_POINTER = 1 /* B is a device pointer (HBM-resident)   */
+};
+
+/* Discretization<double> (include/psp/discretization.hpp:36-110) plus the
+ * DistHelmholtz constructor knobs (include/psp/dist_helmholtz.hpp:27-29). */
+typedef struct {
+    double omega;
+    int nx, ny, nz;
+    double wx, wy, wz;
+    int frontBC, rightBC, backBC, leftBC, topBC;
+    int bx, by, bz;
+    double sigmax, sigmay, sigmaz;
+    double damping;       /* default 7.0 */
+    int planesPerPanel;   /* default 4   */
+    int cutoff;           /* default 12  */
+    int precision;        /* PSP_HIP_FRONTS_FP64 or PSP_HIP_FRONTS_FP32 */
+} psp_hip_desc;
+
+/* Timings mirroring the per-panel gather/fill/ldl report of
+ * initialize_finalize.hpp:153-165. */
+typedef struct {
+    double seconds_total;    /* wall time of psp_hip_factor               */
+    double seconds_fill;     /* front assembly (FillPanelFronts)          */
+    double seconds_ldl;      /* multifrontal LDL^T                        */
+    double seconds_inverse;  /* diagonal-block inversion (SELINV)         */
+    double flops;            /* algorithmic real flops, 8 per complex MAC */
+    double schur_flops;      /* subset spent in Schur-complement GEMMs    */
+    double bytes_factor;     /* bytes of stored factors                   */
+    int num_panels;
+    int num_fronts;          /* fronts over all panels                    */
+} psp_hip_factor_report;
+
+typedef struct {
+    int status;
+    int iterations;          /* GMRES inner steps over all restarts       */
+    int restarts;
+    double final_max_rel;    /* max_r ||b_r - A x_r|| / ||b_r||           */
+    double seconds_total;
+    double seconds_precond;
+    double seconds_multiply;
+    double seconds_arnoldi;
+    int num_precond, num_multiply;
+} psp_hip_solve_report;
+
+typedef struct psp_hip_ctx psp_hip_ctx;
+
+/* DistHelmholtz ctor (dist_helmholtz/main.hpp:12-436): panel split, nested
+ * dissection and symbolic analysis.  Host work only; no device memory is
+ * touched until psp_hip_factor. */
+int psp_hip_create(const psp_hip_desc* desc, psp_hip_ctx** out);
+
+/* ~DistHelmholtz + Finalize (initialize_finalize.hpp:379-396). */
+int psp_hip_destroy(psp_hip_ctx* ctx);
+
+/* Message of the last failing call on this context (never NULL). */
+const char* psp_hip_last_error(const psp_hip_ctx* ctx);
+
+/* Use `stream` (a hipStream_t, or NULL for the context's own stream) for all
+ * later device work of this context. */
+int psp_hip_set_stream(psp_hip_ctx* ctx, void* stream);
+
+/* DistHelmholtz::Initialize (initialize_finalize.hpp:12-377): velocity is a
+ * host array of nx*ny*nz doubles in natural order x + y*nx + z*nx*ny.
+ * panelScheme: PSP_HIP_LDL_1D or PSP_HIP_LDL_SELINV_1D. */
+int psp_hip_factor(psp_hip_ctx* ctx, const double* velocity, int panelScheme,
+                   psp_hip_factor_report* report);
+
+/* DistHelmholtz::Solve (solve.hpp:12-56) = Pull, InternalSolveWithGMRES
+ * (solve.hpp:241-611), Push.  B holds k right-hand sides in grid layout and
+ * is overwritten with the solutions.  flags & PSP_HIP_DEVICE_POINTER: B is
+ * in device memory.  maxIts <= 0: no cap (as the reference). */
+int psp_hip_solve(psp_hip_ctx* ctx, void* B, int k, int m, double relTol,
+                  int maxIts, int flags, psp_hip_solve_report* report);
+
+/* DistHelmholtz::Multiply (solve.hpp:613-679) on device blocks in grid
+ * layout: y := A x. */
+int psp_hip_multiply(psp_hip_ctx* ctx, const void* x, void* y, int k);
+
+/* DistHelmholtz::Precondition (solve.hpp:681-734) in place on a device
+ * block in grid layout. */
+int psp_hip_precondition(psp_hip_ctx* ctx, void* B, int k);
+
+/* DistHelmholtz::SolvePanel (solve.hpp:736-882) in place on a device block
+ * in grid layout (only panel `panel`'s planes are read and written). */
+int psp_hip_solve_panel(psp_hip_ctx* ctx, void* B, int k, int panel);
+
+/* GMRES with a fixed number of inner steps, for parity at a fixed
+ * iteration count; relHist (host, steps doubles) gets max_r rel. residual. */
+int psp_hip_solve_fixed(psp_hip_ctx* ctx, void* B, int k, int m, int steps,
+                        int flags, double* relHist);
+
+/* Adjoint-state gradient of L = 1/2 sum_s sum_r |u_s(x_r) - d_s(x_r)|^2 with
+ * respect to squared slowness m = 1/c^2 (no reference counterpart; the
+ * extension uses the diagonal term of FormGlobalRow,
+ * initialize_finalize.hpp:567-568).  u: device k source solutions (grid
+ * layout); lambda: device adjoint solutions; grad (device, nx*ny*nz doubles)
+ * receives sum_s Re(omega^2 s1 s2 s3 lambda_s u_s), accumulated if
+ * accumulate != 0. */
+int psp_hip_adjoint_gradient(psp_hip_ctx* ctx, const void* u,
+                             const void* lambda, int k, double* grad,
+                             int accumulate);
+
+/* Geometry queries (main.hpp:35-57, 728-821); host only. */
+int psp_hip_num_panels(const psp_hip_ctx* ctx);
+int psp_hip_panel_info(const psp_hip_ctx* ctx, int panel, int* vPanel,
+                       int* depth, int* padding, int* cls);
+/* Symbolic analysis of panel class cls (0 bottom/inner, 1 leftover, 2 top):
+ * per node offset, size, parent, struct size (arrays of
+ * psp_hip_class_num_nodes entries, any may be NULL) and the concatenated
+ * sorted structures. */
+int psp_hip_class_num_nodes(const psp_hip_ctx* ctx, int cls);
+int psp_hip_class_nodes(const psp_hip_ctx* ctx, int cls, int* off, int* size,
+                        int* parent, int* usize, int* structs);
+/* In-panel nested-dissection index (main.hpp:1425-1477). */
+int psp_hip_reordered_index(const psp_hip_ctx* ctx, int x, int y, int vLocal,
+                            int vSize);
+
+/* Device inventory string (host). */
+int psp_hip_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSP_HIP_H */

@@ -1,0 +1,105 @@
+"""ctypes binding of libpsp_hip.so (the C ABI declared in include/psp_hip.h).
+
+The library is loaded from this package directory only; there is no CPU
+fallback.  A missing or stale library raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsp_hip.so")
+
+OK, ERR_INVALID, ERR_SHALLOW, ERR_NAN, ERR_BREAKDOWN, ERR_NOCONV, ERR_STATE, ERR_HIP, ERR_MEMORY = range(9)
+PML, DIRICHLET = 0, 1
+FRONTS_FP64, FRONTS_FP32 = 0, 1
+LDL_1D, LDL_SELINV_1D = 0, 1
+DEVICE_POINTER = 1
+
+# Every entry point include/psp_hip.h declares.
+EXPORTS = (
+    "psp_hip_create", "psp_hip_destroy", "psp_hip_last_error", "psp_hip_set_stream",
+    "psp_hip_factor", "psp_hip_solve", "psp_hip_multiply", "psp_hip_precondition",
+    "psp_hip_solve_panel", "psp_hip_solve_fixed", "psp_hip_adjoint_gradient",
+    "psp_hip_num_panels", "psp_hip_panel_info", "psp_hip_class_num_nodes",
+    "psp_hip_class_nodes", "psp_hip_reordered_index", "psp_hip_device_count",
+)
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("omega", C.c_double),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("wx", C.c_double), ("wy", C.c_double), ("wz", C.c_double),
+        ("frontBC", C.c_int), ("rightBC", C.c_int), ("backBC", C.c_int),
+        ("leftBC", C.c_int), ("topBC", C.c_int),
+        ("bx", C.c_int), ("by", C.c_int), ("bz", C.c_int),
+        ("sigmax", C.c_double), ("sigmay", C.c_double), ("sigmaz", C.c_double),
+        ("damping", C.c_double),
+        ("planesPerPanel", C.c_int),
+        ("cutoff", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class FactorReport(C.Structure):
+    _fields_ = [
+        ("seconds_total", C.c_double), ("seconds_fill", C.c_double),
+        ("seconds_ldl", C.c_double), ("seconds_inverse", C.c_double),
+        ("flops", C.c_double), ("schur_flops", C.c_double),
+        ("bytes_factor", C.c_double),
+        ("num_panels", C.c_int), ("num_fronts", C.c_int),
+    ]
+
+
+class SolveReport(C.Structure):
+    _fields_ = [
+        ("status", C.c_int), ("iterations", C.c_int), ("restarts", C.c_int),
+        ("final_max_rel", C.c_double), ("seconds_total", C.c_double),
+        ("seconds_precond", C.c_double), ("seconds_multiply", C.c_double),
+        ("seconds_arnoldi", C.c_double),
+        ("num_precond", C.c_int), ("num_multiply", C.c_int),
+    ]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libpsp_hip.so and declare every C ABI signature."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C psp_amd/csrc` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    IP = C.POINTER(C.c_int)
+    sig = {
+        "psp_hip_create": (I, [C.POINTER(Desc), C.POINTER(P)]),
+        "psp_hip_destroy": (I, [P]),
+        "psp_hip_last_error": (C.c_char_p, [P]),
+        "psp_hip_set_stream": (I, [P, P]),
+        "psp_hip_factor": (I, [P, P, I, C.POINTER(FactorReport)]),
+        "psp_hip_solve": (I, [P, P, I, I, D, I, I, C.POINTER(SolveReport)]),
+        "psp_hip_multiply": (I, [P, P, P, I]),
+        "psp_hip_precondition": (I, [P, P, I]),
+        "psp_hip_solve_panel": (I, [P, P, I, I]),
+        "psp_hip_solve_fixed": (I, [P, P, I, I, I, I, P]),
+        "psp_hip_adjoint_gradient": (I, [P, P, P, I, P, I]),
+        "psp_hip_num_panels": (I, [P]),
+        "psp_hip_panel_info": (I, [P, I, IP, IP, IP, IP]),
+        "psp_hip_class_num_nodes": (I, [P, I]),
+        "psp_hip_class_nodes": (I, [P, I, P, P, P, P, P]),
+        "psp_hip_reordered_index": (I, [P, I, I, I, I]),
+        "psp_hip_device_count": (I, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L

@@ -1,0 +1,1101 @@
+// HIP kernels for gfx950 (MI355X): front assembly, multifrontal LDL^T,
+// front solves, sweep couplings, the PML 7-point operator and the GMRES
+// vector primitives.  All kernels use 256-thread workgroups (4 wave64).
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace psp_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+// --------------------------------------------------------------------------
+// Complex arithmetic (interleaved re, im).
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T r, T i) { return cx<T>{r, i}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.r + b.r, a.i + b.i}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.r - b.r, a.i - b.i}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a) { return {-a.r, -a.i}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b)
+{
+    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, T s) { return {a.r * s, a.i * s}; }
+// acc + a*b
+template <typename T>
+__device__ __forceinline__ cx<T> cfma(cx<T> a, cx<T> b, cx<T> acc)
+{
+    acc.r = fma(a.r, b.r, acc.r);
+    acc.r = fma(-a.i, b.i, acc.r);
+    acc.i = fma(a.r, b.i, acc.i);
+    acc.i = fma(a.i, b.r, acc.i);
+    return acc;
+}
+// acc - a*b
+template <typename T>
+__device__ __forceinline__ cx<T> cfms(cx<T> a, cx<T> b, cx<T> acc)
+{
+    acc.r = fma(-a.r, b.r, acc.r);
+    acc.r = fma(a.i, b.i, acc.r);
+    acc.i = fma(-a.r, b.i, acc.i);
+    acc.i = fma(-a.i, b.r, acc.i);
+    return acc;
+}
+// Smith's complex division a / b.
+template <typename T>
+__device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b)
+{
+    if (fabs(b.r) >= fabs(b.i)) {
+        const T rr = b.i / b.r, den = b.r + b.i * rr;
+        return {(a.r + a.i * rr) / den, (a.i - a.r * rr) / den};
+    }
+    const T rr = b.r / b.i, den = b.i + b.r * rr;
+    return {(a.r * rr + a.i) / den, (a.i * rr - a.r) / den};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cinv(cx<T> b) { return cdiv(cx<T>{T(1), T(0)}, b); }
+template <typename T>
+__device__ __forceinline__ cx<T> cdivr(cx<T> a, T s) { return {a.r / s, a.i / s}; }
+
+template <typename To, typename Ti>
+__device__ __forceinline__ cx<To> cvt(cx<Ti> a) { return {To(a.r), To(a.i)}; }
+
+template <typename T>
+__device__ __forceinline__ cx<T> czero() { return {T(0), T(0)}; }
+
+// --------------------------------------------------------------------------
+// PML 7-point stencil: FormGlobalRow (initialize_finalize.hpp:527-587) and
+// FormLowerColumnOfNode (:589-640).  out[0] centre, out[1..6] the values of
+// the x-1, x+1, y-1, y+1, v-1, v+1 couplings.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void Stencil(c128 s1L, c128 s1M, c128 s1R, c128 s2L, c128 s2M,
+                                        c128 s2R, c128 s3L, c128 s3M, c128 s3R, c128 k2,
+                                        double hx, double hy, double hz, c128 out[7])
+{
+    const c128 xTop = s2M * s3M;
+    const c128 xTempL = cdiv(xTop, s1L), xTempM = cdiv(xTop, s1M), xTempR = cdiv(xTop, s1R);
+    const c128 xTermL = cdivr(xTempL + xTempM, 2 * hx * hx);
+    const c128 xTermR = cdivr(xTempR + xTempM, 2 * hx * hx);
+    const c128 yTop = s1M * s3M;
+    const c128 yTempL = cdiv(yTop, s2L), yTempM = cdiv(yTop, s2M), yTempR = cdiv(yTop, s2R);
+    const c128 yTermL = cdivr(yTempL + yTempM, 2 * hy * hy);
+    const c128 yTermR = cdivr(yTempR + yTempM, 2 * hy * hy);
+    const c128 zTop = s1M * s2M;
+    const c128 vTempL = cdiv(zTop, s3L), vTempM = cdiv(zTop, s3M), vTempR = cdiv(zTop, s3R);
+    const c128 vTermL = cdivr(vTempL + vTempM, 2 * hz * hz);
+    const c128 vTermR = cdivr(vTempR + vTempM, 2 * hz * hz);
+    out[0] = (xTermL + xTermR + yTermL + yTermR + vTermL + vTermR) - k2 * s1M * s2M * s3M;
+    out[1] = -xTermL;
+    out[2] = -xTermR;
+    out[3] = -yTermL;
+    out[4] = -yTermR;
+    out[5] = -vTermL;
+    out[6] = -vTermR;
+}
+
+// Global operator row at (x, y, v): undamped frequency, physical PML.
+__device__ __forceinline__ void GlobalRow(const StencilArgs& st, double alpha, int x, int y,
+                                          int v, c128 out[7])
+{
+    const double wa = st.omega / alpha;
+    const c128 k2 = {wa * wa, 0.0};
+    Stencil(st.s1[x], st.s1[x + 1], st.s1[x + 2], st.s2[y], st.s2[y + 1], st.s2[y + 2],
+            st.s3[v], st.s3[v + 1], st.s3[v + 2], k2, st.hx, st.hy, st.hz, out);
+}
+
+// --------------------------------------------------------------------------
+// Fill
+// --------------------------------------------------------------------------
+__global__ void k_fill_zero(uint4* p, int64_t n16)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ void k_fill_zero_bytes(unsigned char* p, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+// --------------------------------------------------------------------------
+// Front assembly of the original panel operator (FillPanelFronts,
+// initialize_finalize.hpp:1284-1337): one thread per padded unknown of one
+// panel of the batch; writes the diagonal and the strictly-lower couplings
+// of its column into its front.
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_scatter_orig(
+    int n, int vsize, const int32_t* qxy, const int32_t* qv, const int32_t* qnode,
+    const int32_t* qcol, const int32_t* qrows, const int64_t* nodeL, const int32_t* nodeS,
+    const int32_t* nodeU, const int32_t* panelVOff, const c128* s3art, StencilArgs st,
+    const double* vel, double damping, cx<T>* L, int64_t fstore)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int b = blockIdx.y;
+    const int x = qxy[q] & 0xffff, y = qxy[q] >> 16, vl = qv[q];
+    const int vOffset = panelVOff[b];
+    const int v = vOffset + vl;
+    const int z = st.nz - 1 - v;
+    const double alpha = vel[x + (int64_t)y * st.nx + (int64_t)z * st.nx * st.ny];
+    const c128* s3 = s3art + (int64_t)b * (vsize + 2);
+    const c128 so = {st.omega / alpha, damping / alpha};  // (omega + i damping)/alpha
+    const c128 k2 = so * so;
+    c128 e[7];
+    Stencil(st.s1[x], st.s1[x + 1], st.s1[x + 2], st.s2[y], st.s2[y + 1], st.s2[y + 2],
+            s3[vl], s3[vl + 1], s3[vl + 2], k2, st.hx, st.hy, st.hz, e);
+    const int t = qnode[q], j = qcol[q];
+    const int64_t fs = nodeS[t] + nodeU[t];
+    cx<T>* col = L + (int64_t)b * fstore + nodeL[t] + (int64_t)j * fs;
+    col[j] = cvt<T>(e[0]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int row = qrows[(int64_t)q * 6 + k];
+        if (row >= 0) col[row] = cvt<T>(e[k + 1]);
+    }
+}
+
+// --------------------------------------------------------------------------
+// Extend-add of a child's Schur complement into its parent front.
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_extend_add(const ExtAddItem* items, int start,
+                                                          const int32_t* relinds, cx<T>* L,
+                                                          int64_t fstore, cx<T>* U,
+                                                          int64_t ustore)
+{
+    const ExtAddItem it = items[start + blockIdx.x];
+    const int b = blockIdx.y;
+    const cx<T>* Uc = U + (int64_t)b * ustore + it.uChild;
+    cx<T>* Lp = L + (int64_t)b * fstore + it.lParent;
+    cx<T>* Up = U + (int64_t)b * ustore + it.uParent;
+    const int32_t* rel = relinds + it.relStart;
+    const int ldp = it.sp + it.up;
+    for (int cb = it.col0; cb < it.col1; ++cb) {
+        const int rb = rel[cb];
+        for (int a = cb + threadIdx.x; a < it.uc; a += blockDim.x) {
+            const cx<T> val = Uc[a + (int64_t)cb * it.uc];
+            const int ra = rel[a];
+            if (rb < it.sp) {
+                cx<T>* p = Lp + ra + (int64_t)rb * ldp;
+                *p = *p + val;
+            } else {
+                cx<T>* p = Up + (ra - it.sp) + (int64_t)(rb - it.sp) * it.up;
+                *p = *p + val;
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Blocked LDL^T, step 1: unpivoted LDL^T of the diagonal block in LDS.
+// --------------------------------------------------------------------------
+template <typename T, int NB>
+__global__ __launch_bounds__(kThreads) void k_ldl_diag(const LdlItem* items, int start, cx<T>* L,
+                                                        int64_t fstore)
+{
+    const LdlItem it = items[start + blockIdx.x];
+    cx<T>* F = L + (int64_t)blockIdx.y * fstore + it.l;
+    const int ld = it.ld, j0 = it.j0, m = it.j1 - it.j0;
+    __shared__ cx<T> A[NB][NB + 1];  // A[col][row]
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+        const int r = e % m, c = e / m;
+        if (r >= c) A[c][r] = F[(j0 + r) + (int64_t)(j0 + c) * ld];
+    }
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+        const cx<T> dinv = cinv(A[j][j]);
+        const int mm = m - j - 1;
+        for (int e = threadIdx.x; e < mm * mm; e += blockDim.x) {
+            const int r = j + 1 + e % mm, c = j + 1 + e / mm;
+            if (r >= c) A[c][r] = cfms(A[j][r] * dinv, A[j][c], A[c][r]);
+        }
+        __syncthreads();
+        for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) A[j][r] = A[j][r] * dinv;
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+        const int r = e % m, c = e / m;
+        if (r >= c) F[(j0 + r) + (int64_t)(j0 + c) * ld] = A[c][r];
+    }
+}
+
+// Blocked LDL^T, step 2: rows below the diagonal block,
+// L21 = F21 L11^{-T} D1^{-1}, one thread per row.
+template <typename T, int NB>
+__global__ __launch_bounds__(kThreads) void k_ldl_trsm(const LdlItem* items, int start, cx<T>* L,
+                                                        int64_t fstore)
+{
+    const LdlItem it = items[start + blockIdx.x];
+    cx<T>* F = L + (int64_t)blockIdx.y * fstore + it.l;
+    const int ld = it.ld, j0 = it.j0, m = it.j1 - it.j0;
+    __shared__ cx<T> W[NB][NB];  // W[c][q] = d_q L11(c, q), q < c
+    __shared__ cx<T> dinv[NB];
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+        const int c = e / m, q = e % m;
+        if (q < c)
+            W[c][q] = F[(j0 + q) + (int64_t)(j0 + q) * ld] * F[(j0 + c) + (int64_t)(j0 + q) * ld];
+    }
+    for (int c = threadIdx.x; c < m; c += blockDim.x)
+        dinv[c] = cinv(F[(j0 + c) + (int64_t)(j0 + c) * ld]);
+    __syncthreads();
+    const int r = it.r0 + threadIdx.x;
+    if (r >= it.r1) return;
+    cx<T> x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = c < m ? F[r + (int64_t)(j0 + c) * ld] : czero<T>();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        if (c < m) {
+            cx<T> acc = x[c];
+#pragma unroll
+            for (int q = 0; q < c; ++q) acc = cfms(x[q], W[c][q], acc);
+            x[c] = acc * dinv[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (c < m) F[r + (int64_t)(j0 + c) * ld] = x[c];
+}
+
+// --------------------------------------------------------------------------
+// Batched tile GEMM: C (+)= +/- opA(A) diag(D) opB(B) over a list of
+// 64x64 tiles (see GemmFlags).  Vector FMA path.
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gemm(const GemmDesc* descs, const GemmTile* tiles,
+                                                    int tileStart, cx<T>* b0, int64_t st0,
+                                                    cx<T>* b1, int64_t st1, cx<T>* b2,
+                                                    int64_t st2)
+{
+    constexpr int TM = 64, TK = 16;
+    const GemmTile tl = tiles[tileStart + blockIdx.x];
+    const GemmDesc g = descs[tl.desc];
+    const int bt = blockIdx.y;
+    cx<T>* bases[3] = {b0 + (int64_t)bt * st0, b1 + (int64_t)bt * st1, b2 + (int64_t)bt * st2};
+    const cx<T>* A = bases[g.abuf] + g.a;
+    const cx<T>* B = bases[g.bbuf] + g.b;
+    cx<T>* Cm = bases[g.cbuf] + g.c;
+    const cx<T>* D = bases[g.dbuf] + g.d;
+    const int tm = tl.tm, tn = tl.tn;
+    const int f = g.flags;
+    __shared__ cx<T> As[TK][TM + 1];
+    __shared__ cx<T> Bs[TK][TM + 1];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    cx<T> acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = czero<T>();
+    int kbeg = 0, kend = g.K;
+    if (f & G_BUNIT) kbeg = tn;
+    if (f & G_AUNIT) kend = min(kend, tm + TM);
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const int e = tid + kThreads * l;
+            const int i = e & 63, kk = e >> 6;
+            const int gi = tm + i, gk = k0 + kk;
+            cx<T> v = czero<T>();
+            if (gi < g.M && gk < kend) {
+                if (f & G_AUNIT)
+                    v = gi > gk ? A[gi + (int64_t)gk * g.lda]
+                                : (gi == gk ? cx<T>{T(1), T(0)} : czero<T>());
+                else
+                    v = A[gi + (int64_t)gk * g.lda];
+                if (f & G_HASD) v = v * D[(int64_t)gk * g.dinc];
+            }
+            As[kk][i] = v;
+        }
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            const int e = tid + kThreads * l;
+            int j, kk;
+            if (f & G_BTRANS) {
+                j = e & 63;
+                kk = e >> 6;
+            } else {
+                kk = e & 15;
+                j = e >> 4;
+            }
+            const int gj = tn + j, gk = k0 + kk;
+            cx<T> v = czero<T>();
+            if (gj < g.N && gk < kend) {
+                if (f & G_BTRANS)
+                    v = B[gj + (int64_t)gk * g.ldb];
+                else if (f & G_BUNIT)
+                    v = gk > gj ? B[gk + (int64_t)gj * g.ldb]
+                                : (gk == gj ? cx<T>{T(1), T(0)} : czero<T>());
+                else
+                    v = B[gk + (int64_t)gj * g.ldb];
+            }
+            Bs[kk][j] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK; ++kk) {
+            cx<T> a[4], bb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = As[kk][tx + 16 * q];
+                bb[q] = Bs[kk][ty + 16 * q];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] = cfma(a[p], bb[q], acc[p][q]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = tm + tx + 16 * p, j = tn + ty + 16 * q;
+            if (i >= g.M || j >= g.N) continue;
+            if ((f & G_CLOWER) && i < j) continue;
+            cx<T>* c = Cm + i + (int64_t)j * g.ldc;
+            const cx<T> v = (f & G_NEG) ? -acc[p][q] : acc[p][q];
+            *c = (f & G_BETA0) ? v : (*c + v);
+        }
+}
+
+// Inverse of one unit-lower diagonal block in place (strict lower part).
+template <typename T, int NB>
+__global__ __launch_bounds__(kThreads) void k_inv_diag(const InvItem* items, cx<T>* L,
+                                                        int64_t fstore)
+{
+    const InvItem it = items[blockIdx.x];
+    cx<T>* F = L + (int64_t)blockIdx.y * fstore + it.l;
+    const int ld = it.ld, j0 = it.j0, m = it.j1 - it.j0;
+    __shared__ cx<T> A[NB][NB + 1];  // A[col][row] strictly lower
+    __shared__ cx<T> X[NB][NB + 1];  // X[col][row] inverse
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+        const int r = e % m, c = e / m;
+        A[c][r] = r > c ? F[(j0 + r) + (int64_t)(j0 + c) * ld] : czero<T>();
+    }
+    __syncthreads();
+    const int c = threadIdx.x;
+    if (c < m) {
+        X[c][c] = cx<T>{T(1), T(0)};
+        for (int r = c + 1; r < m; ++r) {
+            cx<T> acc = czero<T>();
+            for (int q = c; q < r; ++q) acc = cfms(A[q][r], X[c][q], acc);
+            X[c][r] = acc;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+        const int r = e % m, cc = e / m;
+        if (r > cc) F[(j0 + r) + (int64_t)(j0 + cc) * ld] = X[cc][r];
+    }
+}
+
+// --------------------------------------------------------------------------
+// Front solves on the padded panel vector Xp (rows x k, interleaved).
+// --------------------------------------------------------------------------
+
+// Forward gather: W = [Xp_own; 0] + extend-add of the children's updates.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveItem* items, int start,
+                                                          const int64_t* nodeW,
+                                                          const int32_t* nodeS,
+                                                          const int32_t* nodeU,
+                                                          const int32_t* nodeUs,
+                                                          const int32_t* relinds,
+                                                          const cx<T>* Xp, cx<T>* W, int k)
+{
+    const SolveItem it = items[start + blockIdx.x];
+    cx<T>* Wt = W + it.w * k;
+    const int fs = it.s + it.u;
+    for (int e = threadIdx.x; e < fs * k; e += blockDim.x) {
+        const int row = e / k;
+        Wt[e] = row < it.s ? Xp[(int64_t)it.off * k + e] : czero<T>();
+    }
+    __syncthreads();
+    const int ch[2] = {it.child0, it.child1};
+    for (int c = 0; c < 2; ++c) {
+        if (ch[c] < 0) continue;
+        const int cs = nodeS[ch[c]], cu = nodeU[ch[c]];
+        const cx<T>* Wc = W + (nodeW[ch[c]] + cs) * k;
+        const int32_t* rel = relinds + nodeUs[ch[c]];
+        for (int e = threadIdx.x; e < cu * k; e += blockDim.x) {
+            const int a = e / k, r = e - a * k;
+            cx<T>* p = Wt + (int64_t)rel[a] * k + r;
+            *p = *p + Wc[e];
+        }
+        __syncthreads();
+    }
+}
+
+// Forward: y = Linv W_top for rows [r0, r1); y goes to Xp_own.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_inv(const SolveItem* items, int start,
+                                                       const cx<T>* L, cx<T>* Xp, const cx<T>* W,
+                                                       int k)
+{
+    constexpr int KC = 8, CH = 32;
+    const SolveItem it = items[start + blockIdx.x];
+    const cx<T>* Lt = L + it.l;
+    const int fs = it.s + it.u;
+    const cx<T>* Wt = W + it.w * k;
+    __shared__ cx<T> ws[CH][KC];
+    const int i = it.r0 + threadIdx.x;
+    for (int r0 = 0; r0 < k; r0 += KC) {
+        const int kc = min(KC, k - r0);
+        cx<T> acc[KC];
+#pragma unroll
+        for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
+        for (int c0 = 0; c0 < it.r1; c0 += CH) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < CH * KC; e += blockDim.x) {
+                const int c = c0 + e / KC, r = e % KC;
+                ws[e / KC][r] = (c < it.s && r < kc) ? Wt[(int64_t)c * k + r0 + r] : czero<T>();
+            }
+            __syncthreads();
+            if (i < it.r1) {
+                const int cend = min(CH, i + 1 - c0);
+                for (int cc = 0; cc < cend; ++cc) {
+                    const int c = c0 + cc;
+                    const cx<T> lv = c == i ? cx<T>{T(1), T(0)} : Lt[i + (int64_t)c * fs];
+#pragma unroll
+                    for (int r = 0; r < KC; ++r) acc[r] = cfma(lv, ws[cc][r], acc[r]);
+                }
+            }
+        }
+        if (i < it.r1) {
+#pragma unroll
+            for (int r = 0; r < KC; ++r)
+                if (r < kc) Xp[(int64_t)(it.off + i) * k + r0 + r] = acc[r];
+        }
+    }
+}
+
+// Forward: W_bot[a] -= L_BL[a,:] y for a in [r0, r1).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fwd_upd(const SolveItem* items, int start,
+                                                       const cx<T>* L, const cx<T>* Xp, cx<T>* W,
+                                                       int k)
+{
+    constexpr int KC = 8, CH = 32;
+    const SolveItem it = items[start + blockIdx.x];
+    const cx<T>* Lt = L + it.l;
+    const int fs = it.s + it.u;
+    cx<T>* Wb = W + (it.w + it.s) * k;
+    __shared__ cx<T> ys[CH][KC];
+    const int a = it.r0 + threadIdx.x;
+    for (int r0 = 0; r0 < k; r0 += KC) {
+        const int kc = min(KC, k - r0);
+        cx<T> acc[KC];
+#pragma unroll
+        for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
+        for (int c0 = 0; c0 < it.s; c0 += CH) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < CH * KC; e += blockDim.x) {
+                const int c = c0 + e / KC, r = e % KC;
+                ys[e / KC][r] =
+                    (c < it.s && r < kc) ? Xp[(int64_t)(it.off + c) * k + r0 + r] : czero<T>();
+            }
+            __syncthreads();
+            if (a < it.r1) {
+                const int cend = min(CH, it.s - c0);
+                for (int cc = 0; cc < cend; ++cc) {
+                    const cx<T> lv = Lt[it.s + a + (int64_t)(c0 + cc) * fs];
+#pragma unroll
+                    for (int r = 0; r < KC; ++r) acc[r] = cfma(lv, ys[cc][r], acc[r]);
+                }
+            }
+        }
+        if (a < it.r1) {
+#pragma unroll
+            for (int r = 0; r < KC; ++r)
+                if (r < kc) {
+                    cx<T>* p = Wb + (int64_t)a * k + r0 + r;
+                    *p = *p - acc[r];
+                }
+        }
+    }
+}
+
+// Wave-level sum of a complex value.
+template <typename T>
+__device__ __forceinline__ cx<T> WaveSum(cx<T> v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        v.r += __shfl_xor(v.r, o, 64);
+        v.i += __shfl_xor(v.i, o, 64);
+    }
+    return v;
+}
+
+// Backward: W_top[j] = y_j / d_j - sum_a L_BL[a][j] Xp[struct[a]], one wave
+// per column j.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_upd(const SolveItem* items, int start,
+                                                       const cx<T>* L, const int32_t* structs,
+                                                       const cx<T>* Xp, cx<T>* W, int k)
+{
+    constexpr int KC = 8;
+    const SolveItem it = items[start + blockIdx.x];
+    const cx<T>* Lt = L + it.l;
+    const int fs = it.s + it.u;
+    cx<T>* Wt = W + it.w * k;
+    const int32_t* st = structs + it.structStart;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = it.r0 + wave; j < it.r1; j += kThreads / 64) {
+        const cx<T>* col = Lt + (int64_t)j * fs;
+        const cx<T> dinv = cinv(col[j]);
+        for (int r0 = 0; r0 < k; r0 += KC) {
+            const int kc = min(KC, k - r0);
+            cx<T> acc[KC];
+#pragma unroll
+            for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
+            for (int a = lane; a < it.u; a += 64) {
+                const cx<T> lv = col[it.s + a];
+                const cx<T>* xs = Xp + (int64_t)st[a] * k + r0;
+#pragma unroll
+                for (int r = 0; r < KC; ++r)
+                    if (r < kc) acc[r] = cfma(lv, xs[r], acc[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < KC; ++r) acc[r] = WaveSum(acc[r]);
+            if (lane == 0) {
+#pragma unroll
+                for (int r = 0; r < KC; ++r)
+                    if (r < kc) {
+                        const cx<T> y = Xp[(int64_t)(it.off + j) * k + r0 + r];
+                        Wt[(int64_t)j * k + r0 + r] = y * dinv - acc[r];
+                    }
+            }
+        }
+    }
+}
+
+// Backward: Xp_own[j] = sum_{i>=j} Linv[i][j] W_top[i], one wave per column.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_bwd_inv(const SolveItem* items, int start,
+                                                       const cx<T>* L, cx<T>* Xp, const cx<T>* W,
+                                                       int k)
+{
+    constexpr int KC = 8;
+    const SolveItem it = items[start + blockIdx.x];
+    const cx<T>* Lt = L + it.l;
+    const int fs = it.s + it.u;
+    const cx<T>* Wt = W + it.w * k;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = it.r0 + wave; j < it.r1; j += kThreads / 64) {
+        const cx<T>* col = Lt + (int64_t)j * fs;
+        for (int r0 = 0; r0 < k; r0 += KC) {
+            const int kc = min(KC, k - r0);
+            cx<T> acc[KC];
+#pragma unroll
+            for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
+            for (int i = j + 1 + lane; i < it.s; i += 64) {
+                const cx<T> lv = col[i];
+                const cx<T>* ts = Wt + (int64_t)i * k + r0;
+#pragma unroll
+                for (int r = 0; r < KC; ++r)
+                    if (r < kc) acc[r] = cfma(lv, ts[r], acc[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < KC; ++r) acc[r] = WaveSum(acc[r]);
+            if (lane == 0) {
+#pragma unroll
+                for (int r = 0; r < KC; ++r)
+                    if (r < kc)
+                        Xp[(int64_t)(it.off + j) * k + r0 + r] =
+                            Wt[(int64_t)j * k + r0 + r] + acc[r];
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Natural-order vector <-> padded panel vector (SolvePanel's copies,
+// solve.hpp:756-778 and :825-847).
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_copy_in(int n, int k, const int32_t* qxy,
+                                                       const int32_t* qv, const c128* B,
+                                                       cx<T>* Xp, int nx, int ny, int nz,
+                                                       int vOffset, int padding)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * k) return;
+    const int q = (int)(e / k), r = (int)(e - (int64_t)q * k);
+    const int vl = qv[q];
+    if (vl < padding) {
+        Xp[e] = czero<T>();
+        return;
+    }
+    const int x = qxy[q] & 0xffff, y = qxy[q] >> 16;
+    const int z = nz - 1 - (vOffset + vl);
+    const int64_t nat = x + (int64_t)y * nx + (int64_t)z * nx * ny;
+    Xp[e] = cvt<T>(B[nat * k + r]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_copy_out(int n, int k, const int32_t* qxy,
+                                                        const int32_t* qv, c128* B,
+                                                        const cx<T>* Xp, int nx, int ny, int nz,
+                                                        int vOffset, int padding, int mode)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * k) return;
+    const int q = (int)(e / k), r = (int)(e - (int64_t)q * k);
+    const int vl = qv[q];
+    if (vl < padding) return;
+    const int x = qxy[q] & 0xffff, y = qxy[q] >> 16;
+    const int z = nz - 1 - (vOffset + vl);
+    const int64_t nat = x + (int64_t)y * nx + (int64_t)z * nx * ny;
+    const c128 v = cvt<double>(Xp[e]);
+    if (mode == 0)
+        B[nat * k + r] = v;
+    else
+        B[nat * k + r] = B[nat * k + r] + v;
+}
+
+// --------------------------------------------------------------------------
+// Sweep couplings between consecutive panels (solve.hpp:884-1047).
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_subdiag(StencilArgs st, const double* vel, c128* B,
+                                                       int k, int v1)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)st.nx * st.ny;
+    if (e >= plane * k) return;
+    const int64_t xy = e / k;
+    const int r = (int)(e - xy * k);
+    const int x = (int)(xy % st.nx), y = (int)(xy / st.nx);
+    const int z = st.nz - 1 - v1;
+    const int64_t nat = xy + (int64_t)z * plane;
+    c128 c[7];
+    GlobalRow(st, vel[nat], x, y, v1, c);
+    // row (x,y,v1) couples to (x,y,v1-1) = natural z+1 with value c[5]
+    B[nat * k + r] = B[nat * k + r] - c[5] * B[(nat + plane) * k + r];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_supdiag_set(StencilArgs st, const double* vel,
+                                                           const c128* B, const int32_t* nat2q,
+                                                           cx<T>* Xp, int k, int v2, int vlLast)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)st.nx * st.ny;
+    if (e >= plane * k) return;
+    const int64_t xy = e / k;
+    const int r = (int)(e - xy * k);
+    const int x = (int)(xy % st.nx), y = (int)(xy / st.nx);
+    const int z = st.nz - 1 - v2;
+    const int64_t nat = xy + (int64_t)z * plane;
+    c128 c[7];
+    GlobalRow(st, vel[nat], x, y, v2, c);
+    // row (x,y,v2) couples to (x,y,v2+1) = natural z-1 with value c[6]
+    const c128 val = -(c[6] * B[(nat - plane) * k + r]);
+    const int q = nat2q[xy + (int64_t)vlLast * plane];
+    Xp[(int64_t)q * k + r] = cvt<T>(val);
+}
+
+// --------------------------------------------------------------------------
+// Multiply (solve.hpp:613-679) in natural order: y = A x or y = b - A x.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_spmv(StencilArgs st, const double* vel,
+                                                    const c128* X, const c128* Bv, c128* Y, int k,
+                                                    int mode)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)st.nx * st.ny;
+    const int64_t N = plane * st.nz;
+    if (e >= N * k) return;
+    const int64_t p = e / k;
+    const int x = (int)(p % st.nx);
+    const int y = (int)((p / st.nx) % st.ny);
+    const int z = (int)(p / plane);
+    const int v = st.nz - 1 - z;
+    c128 c[7];
+    GlobalRow(st, vel[p], x, y, v, c);
+    c128 acc = X[e] * c[0];
+    if (x > 0) acc = cfma(c[1], X[e - k], acc);
+    if (x + 1 < st.nx) acc = cfma(c[2], X[e + k], acc);
+    if (y > 0) acc = cfma(c[3], X[e - (int64_t)st.nx * k], acc);
+    if (y + 1 < st.ny) acc = cfma(c[4], X[e + (int64_t)st.nx * k], acc);
+    if (v > 0) acc = cfma(c[5], X[e + plane * k], acc);
+    if (v + 1 < st.nz) acc = cfma(c[6], X[e - plane * k], acc);
+    Y[e] = mode == 0 ? acc : (Bv[e] - acc);
+}
+
+// dL/dm_i = sum_s Re(omega^2 s1 s2 s3 lambda_s(i) u_s(i)).
+__global__ __launch_bounds__(kThreads) void k_adjoint_grad(StencilArgs st, const c128* U,
+                                                            const c128* Lam, int k, double* grad,
+                                                            int accumulate)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)st.nx * st.ny;
+    if (p >= plane * st.nz) return;
+    const int x = (int)(p % st.nx);
+    const int y = (int)((p / st.nx) % st.ny);
+    const int z = (int)(p / plane);
+    const int v = st.nz - 1 - z;
+    const c128 s = st.s1[x + 1] * st.s2[y + 1] * st.s3[v + 1];
+    double g = 0.0;
+    for (int r = 0; r < k; ++r) {
+        const c128 t = Lam[p * k + r] * U[p * k + r];
+        const c128 w = s * t;
+        g += w.r;
+    }
+    g *= st.omega * st.omega;
+    grad[p] = accumulate ? grad[p] + g : g;
+}
+
+// --------------------------------------------------------------------------
+// Column reductions (Norms / InnerProducts, environment.hpp:177-211).
+// Each workgroup reduces a contiguous point range; thread t < lanes owns
+// column t % k; partials are summed in a fixed order (deterministic).
+// --------------------------------------------------------------------------
+constexpr int kReduceBlocks = 512;
+
+__global__ __launch_bounds__(kThreads) void k_colnorm_part(const c128* X, int64_t np, int k,
+                                                            double* part)
+{
+    __shared__ double sh[kThreads];
+    const int lanes = (kThreads / k) * k, per = kThreads / k;
+    const int t = threadIdx.x;
+    const int64_t chunk = (np + gridDim.x - 1) / gridDim.x;
+    const int64_t p0 = (int64_t)blockIdx.x * chunk, p1 = min(np, p0 + chunk);
+    double acc = 0.0;
+    if (t < lanes) {
+        const int r = t % k;
+        for (int64_t p = p0 + t / k; p < p1; p += per) {
+            const c128 v = X[p * k + r];
+            acc = fma(v.r, v.r, acc);
+            acc = fma(v.i, v.i, acc);
+        }
+    }
+    sh[t] = acc;
+    __syncthreads();
+    if (t < k) {
+        double s = 0.0;
+        for (int i = 0; i < per; ++i) s += sh[t + i * k];
+        part[(int64_t)blockIdx.x * k + t] = s;
+    }
+}
+
+__global__ void k_colnorm_final(const double* part, int nblk, int k, double* out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= k) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * k + r];
+    out[r] = sqrt(s);
+}
+
+__global__ __launch_bounds__(kThreads) void k_coldot_part(const c128* V, const c128* W,
+                                                           int64_t np, int k, c128* part)
+{
+    __shared__ c128 sh[kThreads];
+    const int lanes = (kThreads / k) * k, per = kThreads / k;
+    const int t = threadIdx.x;
+    const int64_t chunk = (np + gridDim.x - 1) / gridDim.x;
+    const int64_t p0 = (int64_t)blockIdx.x * chunk, p1 = min(np, p0 + chunk);
+    c128 acc = {0.0, 0.0};
+    if (t < lanes) {
+        const int r = t % k;
+        for (int64_t p = p0 + t / k; p < p1; p += per) {
+            const c128 v = V[p * k + r], w = W[p * k + r];
+            // conj(v) * w
+            acc.r = fma(v.r, w.r, acc.r);
+            acc.r = fma(v.i, w.i, acc.r);
+            acc.i = fma(v.r, w.i, acc.i);
+            acc.i = fma(-v.i, w.r, acc.i);
+        }
+    }
+    sh[t] = acc;
+    __syncthreads();
+    if (t < k) {
+        c128 s = {0.0, 0.0};
+        for (int i = 0; i < per; ++i) s = s + sh[t + i * k];
+        part[(int64_t)blockIdx.x * k + t] = s;
+    }
+}
+
+__global__ void k_coldot_final(const c128* part, int nblk, int k, c128* out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= k) return;
+    c128 s = {0.0, 0.0};
+    for (int b = 0; b < nblk; ++b) s = s + part[(int64_t)b * k + r];
+    out[r] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_axpy_cols(const c128* alpha, int sign,
+                                                         const c128* V, c128* W, int64_t n, int k)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int r = (int)(e % k);
+    const c128 a = alpha[r];
+    W[e] = sign > 0 ? cfma(a, V[e], W[e]) : cfms(a, V[e], W[e]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_scale_cols(const double* delta, const c128* W,
+                                                          c128* V, int64_t n, int k)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int r = (int)(e % k);
+    const double inv = 1.0 / delta[r];
+    V[e] = W[e] * inv;
+}
+
+__global__ __launch_bounds__(kThreads) void k_update_x(const c128* X0, const c128* V,
+                                                        const c128* Y, int j, int64_t n, int k,
+                                                        int64_t vstride, c128* X)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int r = (int)(e % k);
+    c128 acc = X0[e];
+    for (int i = 0; i <= j; ++i) acc = cfma(Y[i * k + r], V[i * vstride + e], acc);
+    X[e] = acc;
+}
+
+__global__ __launch_bounds__(kThreads) void k_neg(const c128* X, c128* W, int64_t n)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) W[e] = -X[e];
+}
+
+inline unsigned Blocks(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+}  // namespace
+
+// ==========================================================================
+// Launch wrappers
+// ==========================================================================
+void LaunchFillZero(void* p, int64_t bytes, hipStream_t s)
+{
+    if (bytes <= 0) return;
+    const int64_t n16 = bytes / 16;
+    if (n16 > 0) {
+        const unsigned g = (unsigned)std::min<int64_t>(Blocks(n16), 65535 * 4);
+        hipLaunchKernelGGL(k_fill_zero, dim3(g), dim3(kThreads), 0, s, (uint4*)p, n16);
+    }
+    const int64_t rem = bytes - n16 * 16;
+    if (rem > 0)
+        hipLaunchKernelGGL(k_fill_zero_bytes, dim3(1), dim3(kThreads), 0, s,
+                           (unsigned char*)p + n16 * 16, rem);
+}
+
+template <typename T>
+void LaunchScatterOrig(const ClassDev& c, const StencilArgs& st, const double* vel,
+                       double damping, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_scatter_orig<T>), dim3(Blocks(c.n), c.npanels), dim3(kThreads), 0, s,
+                       c.n, c.vsize, c.qxy, c.qv, c.qnode, c.qcol, c.qrows, c.nodeL, c.nodeS,
+                       c.nodeU, c.panelVOff, c.s3art, st, vel, damping, (cx<T>*)c.L, c.fstore);
+}
+
+template <typename T>
+void LaunchExtendAdd(const ClassDev& c, int start, int count, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL((k_extend_add<T>), dim3(count, c.npanels), dim3(kThreads), 0, s,
+                       c.extadd, start, c.relinds, (cx<T>*)c.L, c.fstore, (cx<T>*)c.U, c.ustore);
+}
+
+template <typename T>
+void LaunchLdlDiag(const ClassDev& c, int start, int count, int nb, hipStream_t s)
+{
+    if (count <= 0) return;
+    (void)nb;
+    hipLaunchKernelGGL((k_ldl_diag<T, 32>), dim3(count, c.npanels), dim3(kThreads), 0, s, c.ldl,
+                       start, (cx<T>*)c.L, c.fstore);
+}
+
+template <typename T>
+void LaunchLdlTrsm(const ClassDev& c, int start, int count, int nb, hipStream_t s)
+{
+    if (count <= 0) return;
+    (void)nb;
+    hipLaunchKernelGGL((k_ldl_trsm<T, 32>), dim3(count, c.npanels), dim3(kThreads), 0, s, c.ldl,
+                       start, (cx<T>*)c.L, c.fstore);
+}
+
+template <typename T>
+void LaunchGemm(const ClassDev& c, int tileStart, int tileCount, hipStream_t s)
+{
+    if (tileCount <= 0) return;
+    cx<T>* T0 = (cx<T>*)c.T;
+    hipLaunchKernelGGL((k_gemm<T>), dim3(tileCount, c.npanels), dim3(kThreads), 0, s, c.gemms,
+                       c.tiles, tileStart, (cx<T>*)c.L, c.fstore, (cx<T>*)c.U, c.ustore, T0,
+                       c.tstore);
+}
+
+template <typename T>
+void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s)
+{
+    if (c.ninv <= 0) return;
+    (void)nb;
+    hipLaunchKernelGGL((k_inv_diag<T, 32>), dim3(c.ninv, c.npanels), dim3(kThreads), 0, s, c.inv,
+                       (cx<T>*)c.L, c.fstore);
+}
+
+template <typename T>
+void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, int k,
+                      hipStream_t s)
+{
+    const cx<T>* L = (const cx<T>*)c.L + (int64_t)b * c.fstore;
+    cx<T>* Xp = (cx<T>*)Xpv;
+    cx<T>* W = (cx<T>*)Wv;
+    for (int d = plan.maxDepth; d >= 0; --d) {
+        const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
+        if (lv.gather1 > lv.gather0)
+            hipLaunchKernelGGL((k_fwd_gather<T>), dim3(lv.gather1 - lv.gather0), dim3(kThreads), 0,
+                               s, c.solve, lv.gather0, c.nodeW, c.nodeS, c.nodeU, c.nodeUs,
+                               c.relinds, Xp, W, k);
+        if (lv.fwd1 > lv.fwd0)
+            hipLaunchKernelGGL((k_fwd_inv<T>), dim3(lv.fwd1 - lv.fwd0), dim3(kThreads), 0, s,
+                               c.solve, lv.fwd0, L, Xp, W, k);
+        if (lv.upd1 > lv.upd0)
+            hipLaunchKernelGGL((k_fwd_upd<T>), dim3(lv.upd1 - lv.upd0), dim3(kThreads), 0, s,
+                               c.solve, lv.upd0, L, Xp, W, k);
+    }
+    for (int d = 0; d <= plan.maxDepth; ++d) {
+        const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
+        if (lv.bupd1 > lv.bupd0)
+            hipLaunchKernelGGL((k_bwd_upd<T>), dim3(lv.bupd1 - lv.bupd0), dim3(kThreads), 0, s,
+                               c.solve, lv.bupd0, L, c.structs, Xp, W, k);
+        if (lv.binv1 > lv.binv0)
+            hipLaunchKernelGGL((k_bwd_inv<T>), dim3(lv.binv1 - lv.binv0), dim3(kThreads), 0, s,
+                               c.solve, lv.binv0, L, Xp, W, k);
+    }
+}
+
+template <typename T>
+void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int ny, int nz,
+                  int vOffset, int padding, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_copy_in<T>), dim3(Blocks((int64_t)c.n * k)), dim3(kThreads), 0, s, c.n,
+                       k, c.qxy, c.qv, B, (cx<T>*)Xp, nx, ny, nz, vOffset, padding);
+}
+
+template <typename T>
+void LaunchCopyOut(const ClassDev& c, c128* B, const void* Xp, int k, int nx, int ny, int nz,
+                   int vOffset, int padding, int mode, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_copy_out<T>), dim3(Blocks((int64_t)c.n * k)), dim3(kThreads), 0, s, c.n,
+                       k, c.qxy, c.qv, B, (const cx<T>*)Xp, nx, ny, nz, vOffset, padding, mode);
+}
+
+void LaunchSubdiag(const StencilArgs& st, const double* vel, c128* B, int k, int v1,
+                   hipStream_t s)
+{
+    const int64_t n = (int64_t)st.nx * st.ny * k;
+    hipLaunchKernelGGL(k_subdiag, dim3(Blocks(n)), dim3(kThreads), 0, s, st, vel, B, k, v1);
+}
+
+template <typename T>
+void LaunchSupdiagSet(const ClassDev& c, const StencilArgs& st, const double* vel,
+                      const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s)
+{
+    const int64_t n = (int64_t)st.nx * st.ny * k;
+    hipLaunchKernelGGL((k_supdiag_set<T>), dim3(Blocks(n)), dim3(kThreads), 0, s, st, vel, B,
+                       c.nat2q, (cx<T>*)Xp, k, v2, vlLast);
+}
+
+void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c128* b, c128* y,
+                int k, int mode, hipStream_t s)
+{
+    const int64_t n = (int64_t)st.nx * st.ny * st.nz * k;
+    hipLaunchKernelGGL(k_spmv, dim3(Blocks(n)), dim3(kThreads), 0, s, st, vel, x, b, y, k, mode);
+}
+
+int ReducePartials(int64_t npoints)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kReduceBlocks, (npoints + 255) / 256));
+}
+
+void LaunchColNorm2(const c128* x, int64_t np, int k, double* part, double* out, hipStream_t s)
+{
+    const int g = ReducePartials(np);
+    hipLaunchKernelGGL(k_colnorm_part, dim3(g), dim3(kThreads), 0, s, x, np, k, part);
+    hipLaunchKernelGGL(k_colnorm_final, dim3((k + 63) / 64), dim3(64), 0, s, part, g, k, out);
+}
+
+void LaunchColDot(const c128* v, const c128* w, int64_t np, int k, c128* part, c128* out,
+                  hipStream_t s)
+{
+    const int g = ReducePartials(np);
+    hipLaunchKernelGGL(k_coldot_part, dim3(g), dim3(kThreads), 0, s, v, w, np, k, part);
+    hipLaunchKernelGGL(k_coldot_final, dim3((k + 63) / 64), dim3(64), 0, s, part, g, k, out);
+}
+
+void LaunchAxpyCols(const c128* alpha, int sign, const c128* v, c128* w, int64_t np, int k,
+                    hipStream_t s)
+{
+    const int64_t n = np * k;
+    hipLaunchKernelGGL(k_axpy_cols, dim3(Blocks(n)), dim3(kThreads), 0, s, alpha, sign, v, w, n,
+                       k);
+}
+
+void LaunchScaleCols(const double* delta, const c128* w, c128* v, int64_t np, int k,
+                     hipStream_t s)
+{
+    const int64_t n = np * k;
+    hipLaunchKernelGGL(k_scale_cols, dim3(Blocks(n)), dim3(kThreads), 0, s, delta, w, v, n, k);
+}
+
+void LaunchUpdateX(const c128* x0, const c128* V, const c128* y, int j, int64_t np, int k,
+                   c128* x, hipStream_t s)
+{
+    const int64_t n = np * k;
+    hipLaunchKernelGGL(k_update_x, dim3(Blocks(n)), dim3(kThreads), 0, s, x0, V, y, j, n, k, n,
+                       x);
+}
+
+void LaunchNeg(const c128* x, c128* w, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_neg, dim3(Blocks(n)), dim3(kThreads), 0, s, x, w, n);
+}
+
+void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, int k, double* grad,
+                       int accumulate, hipStream_t s)
+{
+    const int64_t n = (int64_t)st.nx * st.ny * st.nz;
+    hipLaunchKernelGGL(k_adjoint_grad, dim3(Blocks(n)), dim3(kThreads), 0, s, st, u, lam, k, grad,
+                       accumulate);
+}
+
+// Explicit instantiations for complex double and complex float fronts.
+#define PSP_INST(T)                                                                               \
+    template void LaunchScatterOrig<T>(const ClassDev&, const StencilArgs&, const double*, double, \
+                                       hipStream_t);                                              \
+    template void LaunchExtendAdd<T>(const ClassDev&, int, int, hipStream_t);                     \
+    template void LaunchLdlDiag<T>(const ClassDev&, int, int, int, hipStream_t);                  \
+    template void LaunchLdlTrsm<T>(const ClassDev&, int, int, int, hipStream_t);                  \
+    template void LaunchGemm<T>(const ClassDev&, int, int, hipStream_t);                          \
+    template void LaunchInvDiag<T>(const ClassDev&, int, hipStream_t);                            \
+    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, int,  \
+                                      hipStream_t);                                               \
+    template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
+                                  int, hipStream_t);                                              \
+    template void LaunchCopyOut<T>(const ClassDev&, c128*, const void*, int, int, int, int, int,  \
+                                   int, int, hipStream_t);                                        \
+    template void LaunchSupdiagSet<T>(const ClassDev&, const StencilArgs&, const double*,         \
+                                      const c128*, void*, int, int, int, hipStream_t);
+PSP_INST(double)
+PSP_INST(float)
+#undef PSP_INST
+
+}  // namespace psp_amd

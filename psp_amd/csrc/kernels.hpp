@@ -1,0 +1,110 @@
+// Device-side types and kernel launch wrappers shared by the host driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "symbolic.hpp"
+
+namespace psp_amd {
+
+template <typename T>
+struct __attribute__((aligned(2 * sizeof(T)))) cx {
+    T r, i;
+};
+using c64 = cx<float>;
+using c128 = cx<double>;
+
+// Stencil tables: PML profiles s1Inv(x), s2Inv(y), s3Inv(v) for the index
+// range [-1, n] stored at offset +1 (initialize_finalize.hpp:415-525).
+struct StencilArgs {
+    const c128* s1;   // nx + 2
+    const c128* s2;   // ny + 2
+    const c128* s3;   // nz + 2 (global operator)
+    int nx, ny, nz;
+    double hx, hy, hz;
+    double omega;
+};
+
+// Per-class device copy of the static plan.
+struct ClassDev {
+    int n = 0, vsize = 0, nnodes = 0, npanels = 0;
+    int64_t fstore = 0, ustore = 0, tstore = 0, wstore = 0;
+    int32_t *qxy = nullptr, *qv = nullptr, *qnode = nullptr, *qcol = nullptr, *qrows = nullptr;
+    int32_t *nat2q = nullptr;  // (x + y*nx + vl*nx*ny) -> padded ND index
+    int64_t *nodeL = nullptr, *nodeW = nullptr;
+    int32_t *nodeS = nullptr, *nodeU = nullptr, *nodeUs = nullptr;
+    int32_t *structs = nullptr, *relinds = nullptr;
+    ExtAddItem* extadd = nullptr;
+    LdlItem* ldl = nullptr;
+    GemmDesc* gemms = nullptr;
+    GemmTile* tiles = nullptr;
+    InvItem* inv = nullptr;
+    SolveItem* solve = nullptr;
+    int ninv = 0;
+    int32_t* panelVOff = nullptr;   // per batch entry: panel vOffset
+    c128* s3art = nullptr;          // per batch entry: vsize + 2 profile values
+    void* L = nullptr;              // npanels * fstore
+    void* U = nullptr;              // npanels * ustore
+    void* T = nullptr;              // npanels * tstore
+};
+
+// ---- launch wrappers (kernels.hip) ----
+void LaunchFillZero(void* p, int64_t bytes, hipStream_t s);
+
+template <typename T>
+void LaunchScatterOrig(const ClassDev& c, const StencilArgs& st, const double* vel,
+                       double damping, hipStream_t s);
+template <typename T>
+void LaunchExtendAdd(const ClassDev& c, int start, int count, hipStream_t s);
+template <typename T>
+void LaunchLdlDiag(const ClassDev& c, int start, int count, int nb, hipStream_t s);
+template <typename T>
+void LaunchLdlTrsm(const ClassDev& c, int start, int count, int nb, hipStream_t s);
+template <typename T>
+void LaunchGemm(const ClassDev& c, int tileStart, int tileCount, hipStream_t s);
+template <typename T>
+void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s);
+
+// Solve of one panel (batch entry b of class c) on the padded ND vector Xp
+// (n x k, k interleaved) with node work vectors W (wstore x k).
+template <typename T>
+void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xp, void* W,
+                      int k, hipStream_t s);
+// Natural-order <-> padded panel vector.  mode 0: B = Xp, 1: B += Xp.
+template <typename T>
+void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int ny, int nz,
+                  int vOffset, int padding, hipStream_t s);
+template <typename T>
+void LaunchCopyOut(const ClassDev& c, c128* B, const void* Xp, int k, int nx, int ny, int nz,
+                   int vOffset, int padding, int mode, hipStream_t s);
+// B_{i+1}(first plane v1) -= A(v1, v1-1) B_i(plane v1-1)
+void LaunchSubdiag(const StencilArgs& st, const double* vel, c128* B, int k, int v1,
+                   hipStream_t s);
+// Xp(last plane of panel i) := -A(v2, v2+1) B(plane v2+1); rest of Xp is 0.
+template <typename T>
+void LaunchSupdiagSet(const ClassDev& c, const StencilArgs& st, const double* vel,
+                      const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s);
+
+// y = A x (mode 0) or y = b - A x (mode 1), natural order.
+void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c128* b,
+                c128* y, int k, int mode, hipStream_t s);
+
+// Column reductions over N x k interleaved blocks (deterministic two-pass).
+int ReducePartials(int64_t npoints);
+void LaunchColNorm2(const c128* x, int64_t npoints, int k, double* part, double* out,
+                    hipStream_t s);  // out[r] = ||x_r||_2
+void LaunchColDot(const c128* v, const c128* w, int64_t npoints, int k, c128* part,
+                  c128* out, hipStream_t s);  // out[r] = v_r^H w_r
+void LaunchAxpyCols(const c128* alpha, int sign, const c128* v, c128* w, int64_t npoints,
+                    int k, hipStream_t s);  // w += sign*alpha_r v_r
+void LaunchScaleCols(const double* delta, const c128* w, c128* v, int64_t npoints, int k,
+                     hipStream_t s);  // v = w / delta_r
+void LaunchUpdateX(const c128* x0, const c128* V, const c128* y, int j, int64_t npoints, int k,
+                   c128* x, hipStream_t s);  // x = x0 + sum_{i<=j} y(i,r) V_i
+void LaunchNeg(const c128* x, c128* w, int64_t n, hipStream_t s);
+void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, int k,
+                       double* grad, int accumulate, hipStream_t s);
+
+}  // namespace psp_amd

@@ -1,0 +1,825 @@
+// C ABI implementation: DistHelmholtz's setup, Initialize and Solve on one
+// MI355X.  The whole grid, every panel's factors and the Krylov basis stay in
+// HBM; only the small GMRES least-squares work (Givens rotations, the
+// triangular solve of the Hessenberg matrix) runs on the host, in FP64,
+// exactly as InternalSolveWithGMRES (solve.hpp:241-611) does it.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "psp_hip.h"
+#include "symbolic.hpp"
+
+using namespace psp_amd;
+using zd = std::complex<double>;
+
+namespace {
+
+double Now()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct psp_hip_ctx {
+    psp_hip_desc desc{};
+    Plan plan;
+    std::string err;
+    bool devReady = false;
+    bool factored = false;
+    hipStream_t stream = nullptr;
+    hipStream_t own = nullptr;
+    std::vector<void*> allocs;  // plan + factor storage
+    double* vel = nullptr;
+    c128 *s1 = nullptr, *s2 = nullptr, *s3 = nullptr;
+    ClassDev dev[3];
+    std::vector<int> panelBatch;   // panel -> batch index inside its class
+    std::vector<std::vector<int>> classPanels;
+    // solve workspace
+    int wsK = 0, wsM = 0;
+    std::vector<void*> wsAllocs;
+    void *Xp = nullptr, *W = nullptr;
+    c128 *V = nullptr, *x = nullptr, *x0 = nullptr, *w = nullptr, *b = nullptr;
+    double *partD = nullptr, *normD = nullptr;
+    c128 *partC = nullptr, *Hcol = nullptr, *ydev = nullptr;
+};
+
+namespace {
+
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);            \
+            return e_ == hipErrorOutOfMemory ? PSP_HIP_ERR_MEMORY : PSP_HIP_ERR_HIP;   \
+        }                                                                              \
+    } while (0)
+
+int Fail(psp_hip_ctx* c, int code, const std::string& m)
+{
+    c->err = m;
+    return code;
+}
+
+template <typename X>
+int Upload(psp_hip_ctx* c, X** dst, const std::vector<X>& src)
+{
+    *dst = nullptr;
+    if (src.empty()) return PSP_HIP_OK;
+    void* p = nullptr;
+    HIPCHK(c, hipMalloc(&p, sizeof(X) * src.size()));
+    c->allocs.push_back(p);
+    HIPCHK(c, hipMemcpy(p, src.data(), sizeof(X) * src.size(), hipMemcpyHostToDevice));
+    *dst = (X*)p;
+    return PSP_HIP_OK;
+}
+
+int Alloc(psp_hip_ctx* c, void** p, size_t bytes, std::vector<void*>& list)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    HIPCHK(c, hipMalloc(p, bytes));
+    list.push_back(*p);
+    return PSP_HIP_OK;
+}
+
+// PML profiles (initialize_finalize.hpp:415-525).
+zd ProfileValue(double sigma, int b, double h, double omega, double delta)
+{
+    const double eta = b * h;
+    return zd(1.0, (sigma / eta) * (delta / b) * (delta / b) * (2 * M_PI / omega));
+}
+
+zd S1Inv(const psp_hip_desc& d, double hx, int x)
+{
+    if (x < d.bx - 1 && d.frontBC == PSP_HIP_PML)
+        return ProfileValue(d.sigmax, d.bx, hx, d.omega, (d.bx - 1) - x);
+    if (x > d.nx - d.bx && d.backBC == PSP_HIP_PML)
+        return ProfileValue(d.sigmax, d.bx, hx, d.omega, x - (d.nx - d.bx));
+    return 1.0;
+}
+
+zd S2Inv(const psp_hip_desc& d, double hy, int y)
+{
+    if (y < d.by - 1 && d.leftBC == PSP_HIP_PML)
+        return ProfileValue(d.sigmay, d.by, hy, d.omega, (d.by - 1) - y);
+    if (y > d.ny - d.by && d.rightBC == PSP_HIP_PML)
+        return ProfileValue(d.sigmay, d.by, hy, d.omega, y - (d.ny - d.by));
+    return 1.0;
+}
+
+zd S3Inv(const psp_hip_desc& d, double hz, int v, int vOffset, bool artificial)
+{
+    const int lim = artificial ? vOffset + d.bz - 1 : d.bz - 1;
+    if (v < lim) return ProfileValue(d.sigmaz, d.bz, hz, d.omega, lim - v);
+    if (v > d.nz - d.bz && d.topBC == PSP_HIP_PML)
+        return ProfileValue(d.sigmaz, d.bz, hz, d.omega, v - (d.nz - d.bz));
+    return 1.0;
+}
+
+std::vector<c128> ToC128(const std::vector<zd>& v)
+{
+    std::vector<c128> o(v.size());
+    for (size_t i = 0; i < v.size(); ++i) o[i] = c128{v[i].real(), v[i].imag()};
+    return o;
+}
+
+StencilArgs Stencil(const psp_hip_ctx* c)
+{
+    StencilArgs st;
+    st.s1 = c->s1;
+    st.s2 = c->s2;
+    st.s3 = c->s3;
+    st.nx = c->desc.nx;
+    st.ny = c->desc.ny;
+    st.nz = c->desc.nz;
+    st.hx = c->desc.wx / (c->desc.nx + 1);
+    st.hy = c->desc.wy / (c->desc.ny + 1);
+    st.hz = c->desc.wz / (c->desc.nz + 1);
+    st.omega = c->desc.omega;
+    return st;
+}
+
+size_t FrontBytes(const psp_hip_ctx* c) { return c->desc.precision == PSP_HIP_FRONTS_FP32 ? 8 : 16; }
+
+// One-time device setup: upload the static plan and allocate factor storage.
+int EnsureDevice(psp_hip_ctx* c)
+{
+    if (c->devReady) return PSP_HIP_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return Fail(c, PSP_HIP_ERR_HIP, "no HIP device available");
+    if (!c->stream) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+        c->stream = c->own;
+    }
+    const Geometry& g = c->plan.geo;
+    c->classPanels.assign(3, {});
+    c->panelBatch.assign(g.numPanels, 0);
+    for (int p = 0; p < g.numPanels; ++p) {
+        const int cl = g.cls(p);
+        c->panelBatch[p] = (int)c->classPanels[cl].size();
+        c->classPanels[cl].push_back(p);
+    }
+    const psp_hip_desc& d = c->desc;
+    const double hx = d.wx / (d.nx + 1), hy = d.wy / (d.ny + 1), hz = d.wz / (d.nz + 1);
+    std::vector<zd> s1(d.nx + 2), s2(d.ny + 2), s3(d.nz + 2);
+    for (int x = -1; x <= d.nx; ++x) s1[x + 1] = S1Inv(d, hx, x);
+    for (int y = -1; y <= d.ny; ++y) s2[y + 1] = S2Inv(d, hy, y);
+    for (int v = -1; v <= d.nz; ++v) s3[v + 1] = S3Inv(d, hz, v, 0, false);
+    int st;
+    if ((st = Upload(c, &c->s1, ToC128(s1)))) return st;
+    if ((st = Upload(c, &c->s2, ToC128(s2)))) return st;
+    if ((st = Upload(c, &c->s3, ToC128(s3)))) return st;
+    void* vp = nullptr;
+    if ((st = Alloc(c, &vp, sizeof(double) * (size_t)d.nx * d.ny * d.nz, c->allocs))) return st;
+    c->vel = (double*)vp;
+
+    const size_t eb = FrontBytes(c);
+    for (int cl = 0; cl < 3; ++cl) {
+        if (!c->plan.present[cl]) continue;
+        const ClassPlan& P = c->plan.cls[cl];
+        ClassDev& D = c->dev[cl];
+        D.n = P.n;
+        D.vsize = P.vsize;
+        D.nnodes = (int)P.nodes.size();
+        D.npanels = (int)c->classPanels[cl].size();
+        D.fstore = P.fstore;
+        D.ustore = P.ustore;
+        D.tstore = P.tstore;
+        D.wstore = P.wstore;
+        D.ninv = (int)P.inv.size();
+        std::vector<int64_t> nL(D.nnodes), nW(D.nnodes);
+        std::vector<int32_t> nS(D.nnodes), nU(D.nnodes), nUs(D.nnodes);
+        for (int t = 0; t < D.nnodes; ++t) {
+            nL[t] = P.nodes[t].loff;
+            nW[t] = P.nodes[t].woff;
+            nS[t] = P.nodes[t].size;
+            nU[t] = P.nodes[t].usize;
+            nUs[t] = (int32_t)P.nodes[t].ustart;
+        }
+        std::vector<int32_t> nat2q(P.n);
+        for (int q = 0; q < P.n; ++q) {
+            const int x = P.qxy[q] & 0xffff, y = P.qxy[q] >> 16;
+            nat2q[x + y * d.nx + P.qv[q] * d.nx * d.ny] = q;
+        }
+        std::vector<int32_t> vOff(D.npanels);
+        std::vector<zd> s3a((size_t)D.npanels * (P.vsize + 2));
+        for (int b = 0; b < D.npanels; ++b) {
+            const int p = c->classPanels[cl][b];
+            vOff[b] = g.vOffset(p);
+            for (int vl = -1; vl <= P.vsize; ++vl)
+                s3a[(size_t)b * (P.vsize + 2) + vl + 1] =
+                    S3Inv(d, hz, vOff[b] + vl, vOff[b], true);
+        }
+        if ((st = Upload(c, &D.qxy, P.qxy)) || (st = Upload(c, &D.qv, P.qv)) ||
+            (st = Upload(c, &D.qnode, P.qnode)) || (st = Upload(c, &D.qcol, P.qcol)) ||
+            (st = Upload(c, &D.qrows, P.qrows)) || (st = Upload(c, &D.nat2q, nat2q)) ||
+            (st = Upload(c, &D.nodeL, nL)) || (st = Upload(c, &D.nodeW, nW)) ||
+            (st = Upload(c, &D.nodeS, nS)) || (st = Upload(c, &D.nodeU, nU)) ||
+            (st = Upload(c, &D.nodeUs, nUs)) || (st = Upload(c, &D.structs, P.structs)) ||
+            (st = Upload(c, &D.relinds, P.relinds)) || (st = Upload(c, &D.extadd, P.extadd)) ||
+            (st = Upload(c, &D.ldl, P.ldl)) || (st = Upload(c, &D.gemms, P.gemms)) ||
+            (st = Upload(c, &D.tiles, P.tiles)) || (st = Upload(c, &D.inv, P.inv)) ||
+            (st = Upload(c, &D.solve, P.solveItems)) || (st = Upload(c, &D.panelVOff, vOff)))
+            return st;
+        std::vector<c128> s3c = ToC128(s3a);
+        if ((st = Upload(c, &D.s3art, s3c))) return st;
+        if ((st = Alloc(c, &D.L, eb * (size_t)D.npanels * D.fstore, c->allocs))) return st;
+        if ((st = Alloc(c, &D.U, eb * (size_t)D.npanels * D.ustore, c->allocs))) return st;
+        if ((st = Alloc(c, &D.T, eb * (size_t)D.npanels * D.tstore, c->allocs))) return st;
+    }
+    c->devReady = true;
+    return PSP_HIP_OK;
+}
+
+void FreeWorkspace(psp_hip_ctx* c)
+{
+    for (void* p : c->wsAllocs) (void)hipFree(p);
+    c->wsAllocs.clear();
+    c->wsK = c->wsM = 0;
+}
+
+int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
+{
+    if (c->wsK == k && c->wsM >= m) return PSP_HIP_OK;
+    FreeWorkspace(c);
+    const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
+    const size_t eb = FrontBytes(c);
+    int64_t maxn = 0, maxw = 0;
+    for (int cl = 0; cl < 3; ++cl)
+        if (c->plan.present[cl]) {
+            maxn = std::max<int64_t>(maxn, c->plan.cls[cl].n);
+            maxw = std::max<int64_t>(maxw, c->plan.cls[cl].wstore);
+        }
+    int st;
+    void* p;
+    if ((st = Alloc(c, &c->Xp, eb * maxn * k, c->wsAllocs))) return st;
+    if ((st = Alloc(c, &c->W, eb * maxw * k, c->wsAllocs))) return st;
+    if ((st = Alloc(c, &p, sizeof(c128) * N * k * m, c->wsAllocs))) return st;
+    c->V = (c128*)p;
+    if ((st = Alloc(c, &p, sizeof(c128) * N * k, c->wsAllocs))) return st;
+    c->x = (c128*)p;
+    if ((st = Alloc(c, &p, sizeof(c128) * N * k, c->wsAllocs))) return st;
+    c->x0 = (c128*)p;
+    if ((st = Alloc(c, &p, sizeof(c128) * N * k, c->wsAllocs))) return st;
+    c->w = (c128*)p;
+    if ((st = Alloc(c, &p, sizeof(c128) * N * k, c->wsAllocs))) return st;
+    c->b = (c128*)p;
+    const int g = ReducePartials(N);
+    if ((st = Alloc(c, &p, sizeof(double) * g * k, c->wsAllocs))) return st;
+    c->partD = (double*)p;
+    if ((st = Alloc(c, &p, sizeof(c128) * g * k, c->wsAllocs))) return st;
+    c->partC = (c128*)p;
+    if ((st = Alloc(c, &p, sizeof(double) * k, c->wsAllocs))) return st;
+    c->normD = (double*)p;
+    if ((st = Alloc(c, &p, sizeof(c128) * (m + 1) * k, c->wsAllocs))) return st;
+    c->Hcol = (c128*)p;
+    if ((st = Alloc(c, &p, sizeof(c128) * (m + 1) * k, c->wsAllocs))) return st;
+    c->ydev = (c128*)p;
+    c->wsK = k;
+    c->wsM = m;
+    return PSP_HIP_OK;
+}
+
+template <typename T>
+int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv)
+{
+    const StencilArgs st = Stencil(c);
+    hipStream_t s = c->stream;
+    hipEvent_t e[4];
+    for (auto& ev : e) HIPCHK(c, hipEventCreate(&ev));
+    float acc[3] = {0, 0, 0};
+    for (int cl = 0; cl < 3; ++cl) {
+        if (!c->plan.present[cl]) continue;
+        const ClassPlan& P = c->plan.cls[cl];
+        const ClassDev& D = c->dev[cl];
+        const int nb = c->plan.blockNb;
+        HIPCHK(c, hipEventRecord(e[0], s));
+        LaunchFillZero(D.L, (int64_t)sizeof(cx<T>) * D.npanels * D.fstore, s);
+        LaunchFillZero(D.U, (int64_t)sizeof(cx<T>) * D.npanels * D.ustore, s);
+        LaunchScatterOrig<T>(D, st, c->vel, c->desc.damping, s);
+        HIPCHK(c, hipEventRecord(e[1], s));
+        for (int dpt = P.maxDepth; dpt >= 0; --dpt) {
+            const ClassPlan::LevelSteps& S = P.steps[dpt];
+            for (int slot = 0; slot < 2; ++slot)
+                for (const auto& r : S.extadd[slot]) LaunchExtendAdd<T>(D, r.first, r.second, s);
+            for (size_t kb = 0; kb < S.ldlDiag.size(); ++kb) {
+                LaunchLdlDiag<T>(D, S.ldlDiag[kb].first, S.ldlDiag[kb].second, nb, s);
+                LaunchLdlTrsm<T>(D, S.ldlTrsm[kb].first, S.ldlTrsm[kb].second, nb, s);
+                LaunchGemm<T>(D, S.trail[kb].tileStart, S.trail[kb].tileCount, s);
+            }
+            LaunchGemm<T>(D, S.schur.tileStart, S.schur.tileCount, s);
+        }
+        HIPCHK(c, hipEventRecord(e[2], s));
+        LaunchInvDiag<T>(D, nb, s);
+        for (const GemmLaunch& gl : P.invGemm) LaunchGemm<T>(D, gl.tileStart, gl.tileCount, s);
+        HIPCHK(c, hipEventRecord(e[3], s));
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventSynchronize(e[3]));
+        float ms;
+        HIPCHK(c, hipEventElapsedTime(&ms, e[0], e[1]));
+        acc[0] += ms;
+        HIPCHK(c, hipEventElapsedTime(&ms, e[1], e[2]));
+        acc[1] += ms;
+        HIPCHK(c, hipEventElapsedTime(&ms, e[2], e[3]));
+        acc[2] += ms;
+    }
+    for (auto& ev : e) (void)hipEventDestroy(ev);
+    *tFill = acc[0] * 1e-3;
+    *tLdl = acc[1] * 1e-3;
+    *tInv = acc[2] * 1e-3;
+    return PSP_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sweep: Precondition (solve.hpp:681-734) on a natural-order device block.
+// ---------------------------------------------------------------------------
+template <typename T>
+void PanelForward(psp_hip_ctx* c, c128* B, int k, int p)
+{
+    const Geometry& g = c->plan.geo;
+    const int cl = g.cls(p);
+    const ClassDev& D = c->dev[cl];
+    hipStream_t s = c->stream;
+    LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s);
+    LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s);
+}
+
+template <typename T>
+void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p)
+{
+    // ExtractPanel + MultiplySuperdiagonal + SolvePanel + UpdatePanel
+    // (solve.hpp:725-733): B_i still holds Z, so the solve of the coupling
+    // term is added onto it.
+    const Geometry& g = c->plan.geo;
+    const int cl = g.cls(p);
+    const ClassDev& D = c->dev[cl];
+    hipStream_t s = c->stream;
+    LaunchFillZero(c->Xp, (int64_t)sizeof(cx<T>) * D.n * k, s);
+    const int v2 = g.panelV(p) + g.depth(p) - 1;
+    const int vlLast = g.padding(p) + g.depth(p) - 1;
+    LaunchSupdiagSet<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s);
+    LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 1, s);
+}
+
+template <typename T>
+void Sweep(psp_hip_ctx* c, c128* B, int k)
+{
+    const Geometry& g = c->plan.geo;
+    const StencilArgs st = Stencil(c);
+    for (int i = 0; i < g.numPanels - 1; ++i) {
+        PanelForward<T>(c, B, k, i);
+        LaunchSubdiag(st, c->vel, B, k, g.panelV(i + 1), c->stream);
+    }
+    PanelForward<T>(c, B, k, g.numPanels - 1);
+    for (int i = g.numPanels - 2; i >= 0; --i) PanelBackward<T>(c, B, k, i);
+}
+
+void Precondition(psp_hip_ctx* c, c128* B, int k)
+{
+    if (c->desc.precision == PSP_HIP_FRONTS_FP32)
+        Sweep<float>(c, B, k);
+    else
+        Sweep<double>(c, B, k);
+}
+
+int Norms(psp_hip_ctx* c, const c128* X, int k, std::vector<double>& out)
+{
+    const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
+    LaunchColNorm2(X, N, k, c->partD, c->normD, c->stream);
+    out.resize(k);
+    HIPCHK(c, hipMemcpyAsync(out.data(), c->normD, sizeof(double) * k, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSP_HIP_OK;
+}
+
+bool HasNaN(const std::vector<double>& v)
+{
+    for (double a : v)
+        if (a != a) return true;
+    return false;
+}
+
+// lapack::ComputeGivens (solve.hpp:486): zlartg semantics,
+// [c s; -conj(s) c] [f; g] = [r; 0].
+void Givens(zd f, zd g, double& c, zd& s, zd& r)
+{
+    if (g == 0.0) {
+        c = 1.0;
+        s = 0.0;
+        r = f;
+        return;
+    }
+    if (f == 0.0) {
+        c = 0.0;
+        s = std::conj(g) / std::abs(g);
+        r = std::abs(g);
+        return;
+    }
+    const double af = std::abs(f), ag = std::abs(g);
+    const double dd = std::hypot(af, ag);
+    const zd ph = f / af;
+    c = af / dd;
+    s = ph * std::conj(g) / dd;
+    r = ph * dd;
+}
+
+struct GmresOut {
+    int it = 0, restarts = 0;
+    double maxRel = 0;
+    double tP = 0, tM = 0, tA = 0;
+    int nP = 0, nM = 0;
+};
+
+// InternalSolveWithGMRES (solve.hpp:241-611): k independent GMRES(m) sharing
+// operator applications; x recomputed as x0 + V y every step; true residual
+// every step; stop when the largest relative residual is below relTol.
+// steps > 0: run exactly that many inner steps and record the history.
+int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, double* hist,
+          GmresOut& o)
+{
+    const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
+    const int64_t n = N * k;
+    hipStream_t s = c->stream;
+    const StencilArgs st = Stencil(c);
+    std::vector<double> orig, beta, delta, resid;
+    std::vector<zd> H((size_t)m * m * k), z((size_t)(m + 1) * k), sn((size_t)m * k);
+    std::vector<double> cs((size_t)m * k);
+    std::vector<c128> hc((size_t)(m + 1) * k), yh((size_t)(m + 1) * k);
+    int rc;
+
+    LaunchFillZero(c->x, sizeof(c128) * n, s);
+    HIPCHK(c, hipMemcpyAsync(c->w, c->b, sizeof(c128) * n, hipMemcpyDeviceToDevice, s));
+    if ((rc = Norms(c, c->w, k, orig))) return rc;
+    if (HasNaN(orig)) return Fail(c, PSP_HIP_ERR_NAN, "Original residual norms had a NaN");
+
+    bool converged = false;
+    while (!converged) {
+        HIPCHK(c, hipMemcpyAsync(c->x0, c->x, sizeof(c128) * n, hipMemcpyDeviceToDevice, s));
+        double t0 = Now();
+        Precondition(c, c->w, k);
+        if ((rc = Norms(c, c->w, k, beta))) return rc;
+        o.tP += Now() - t0;
+        ++o.nP;
+        if (HasNaN(beta)) return Fail(c, PSP_HIP_ERR_NAN, "beta list had a NaN");
+        LaunchScaleCols(c->normD, c->w, c->V, N, k, s);
+        std::fill(z.begin(), z.end(), zd(0));
+        for (int r = 0; r < k; ++r) z[(size_t)r * (m + 1)] = beta[r];
+        for (int j = 0; j < m; ++j) {
+            c128* vj = c->V + (int64_t)j * n;
+            t0 = Now();
+            LaunchSpmv(st, c->vel, vj, nullptr, c->w, k, 0, s);
+            HIPCHK(c, hipStreamSynchronize(s));
+            o.tM += Now() - t0;
+            ++o.nM;
+            t0 = Now();
+            Precondition(c, c->w, k);
+            HIPCHK(c, hipStreamSynchronize(s));
+            o.tP += Now() - t0;
+            ++o.nP;
+            t0 = Now();
+            // Arnoldi by modified Gram-Schmidt (solve.hpp:382-399)
+            for (int i = 0; i <= j; ++i) {
+                const c128* vi = c->V + (int64_t)i * n;
+                LaunchColDot(vi, c->w, N, k, c->partC, c->Hcol + (int64_t)i * k, s);
+                LaunchAxpyCols(c->Hcol + (int64_t)i * k, -1, vi, c->w, N, k, s);
+            }
+            if ((rc = Norms(c, c->w, k, delta))) return rc;
+            HIPCHK(c, hipMemcpyAsync(hc.data(), c->Hcol, sizeof(c128) * (j + 1) * k,
+                                     hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            o.tA += Now() - t0;
+            if (HasNaN(delta)) return Fail(c, PSP_HIP_ERR_NAN, "delta list had a NaN");
+            for (int r = 0; r < k; ++r)
+                if (delta[r] == 0.0)
+                    return Fail(c, PSP_HIP_ERR_BREAKDOWN,
+                                "GMRES halted due to a (usually) lucky breakdown");
+            if (j + 1 != m) LaunchScaleCols(c->normD, c->w, c->V + (int64_t)(j + 1) * n, N, k, s);
+            for (int r = 0; r < k; ++r) {
+                zd* Hr = H.data() + (size_t)r * m * m;  // H(i,j) = Hr[i + j*m]
+                for (int i = 0; i <= j; ++i) {
+                    const c128 a = hc[(size_t)i * k + r];
+                    Hr[i + (size_t)j * m] = zd(a.r, a.i);
+                }
+                for (int i = 0; i < j; ++i) {  // previous rotations (:442-456)
+                    const double cc = cs[i + (size_t)r * m];
+                    const zd sv = sn[i + (size_t)r * m];
+                    const zd e0 = Hr[i + (size_t)j * m], e1 = Hr[i + 1 + (size_t)j * m];
+                    Hr[i + (size_t)j * m] = cc * e0 + sv * e1;
+                    Hr[i + 1 + (size_t)j * m] = -std::conj(sv) * e0 + cc * e1;
+                }
+                const zd ejj = Hr[j + (size_t)j * m];
+                if (ejj != ejj) return Fail(c, PSP_HIP_ERR_NAN, "H(j,j) was NaN");
+                double cc;
+                zd sv, rho;
+                Givens(ejj, zd(delta[r]), cc, sv, rho);
+                if (cc != cc || sv != sv || rho != rho)
+                    return Fail(c, PSP_HIP_ERR_NAN, "Givens rotation was NaN");
+                Hr[j + (size_t)j * m] = rho;
+                cs[j + (size_t)r * m] = cc;
+                sn[j + (size_t)r * m] = sv;
+                zd* zr = z.data() + (size_t)r * (m + 1);
+                const zd zj = zr[j], zj1 = zr[j + 1];
+                zr[j] = cc * zj + sv * zj1;
+                zr[j + 1] = -std::conj(sv) * zj + cc * zj1;
+                std::vector<zd> y(zr, zr + j + 1);  // Trsv(UPPER, NORMAL, NON_UNIT)
+                for (int i = j; i >= 0; --i) {
+                    zd a = y[i];
+                    for (int l = i + 1; l <= j; ++l) a -= Hr[i + (size_t)l * m] * y[l];
+                    y[i] = a / Hr[i + (size_t)i * m];
+                }
+                for (int i = 0; i <= j; ++i) yh[(size_t)i * k + r] = c128{y[i].real(), y[i].imag()};
+            }
+            HIPCHK(c, hipMemcpyAsync(c->ydev, yh.data(), sizeof(c128) * (j + 1) * k,
+                                     hipMemcpyHostToDevice, s));
+            LaunchUpdateX(c->x0, c->V, c->ydev, j, N, k, c->x, s);  // x := x0 + V y
+            t0 = Now();
+            LaunchSpmv(st, c->vel, c->x, c->b, c->w, k, 1, s);  // w := b - A x
+            HIPCHK(c, hipStreamSynchronize(s));
+            o.tM += Now() - t0;
+            ++o.nM;
+            if ((rc = Norms(c, c->w, k, resid))) return rc;
+            if (HasNaN(resid)) return Fail(c, PSP_HIP_ERR_NAN, "resid norm list has NaN");
+            double mx = 0;
+            for (int r = 0; r < k; ++r) mx = std::max(mx, resid[r] / orig[r]);
+            o.maxRel = mx;
+            if (hist && o.it < steps) hist[o.it] = mx;
+            ++o.it;
+            if (steps > 0) {
+                if (o.it >= steps) {
+                    converged = true;
+                    break;
+                }
+            } else if (mx < relTol) {
+                converged = true;
+                break;
+            }
+            if (maxIts > 0 && o.it >= maxIts)
+                return Fail(c, PSP_HIP_ERR_NOCONV, "GMRES reached the iteration cap");
+        }
+        ++o.restarts;
+    }
+    return PSP_HIP_OK;
+}
+
+int CheckSolveArgs(psp_hip_ctx* c, const void* B, int k, int m)
+{
+    if (!c) return PSP_HIP_ERR_INVALID;
+    if (!c->factored) return Fail(c, PSP_HIP_ERR_STATE, "Solve called before Initialize");
+    if (!B) return Fail(c, PSP_HIP_ERR_INVALID, "null right-hand side");
+    if (k < 1 || k > 256) return Fail(c, PSP_HIP_ERR_INVALID, "number of RHS must be in [1, 256]");
+    if (m < 1) return Fail(c, PSP_HIP_ERR_INVALID, "GMRES restart length must be >= 1");
+    return PSP_HIP_OK;
+}
+
+int RunSolve(psp_hip_ctx* c, void* B, int k, int m, double relTol, int maxIts, int flags,
+             int steps, double* hist, psp_hip_solve_report* rep)
+{
+    int rc = CheckSolveArgs(c, B, k, m);
+    if (rc) return rc;
+    const double t0 = Now();
+    if ((rc = EnsureWorkspace(c, k, m))) return rc;
+    const int64_t n = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz * k;
+    const bool dev = flags & PSP_HIP_DEVICE_POINTER;
+    // PullRightHandSides: natural order is the working order on the GPU.
+    HIPCHK(c, hipMemcpyAsync(c->b, B, sizeof(c128) * n,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    GmresOut o;
+    const int st = Gmres(c, k, m, relTol, maxIts, steps, hist, o);
+    // PushRightHandSides: the solution (also on breakdown, cap or NaN).
+    HIPCHK(c, hipMemcpyAsync(B, c->x, sizeof(c128) * n,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (rep) {
+        rep->status = st;
+        rep->iterations = o.it;
+        rep->restarts = o.restarts;
+        rep->final_max_rel = o.maxRel;
+        rep->seconds_total = Now() - t0;
+        rep->seconds_precond = o.tP;
+        rep->seconds_multiply = o.tM;
+        rep->seconds_arnoldi = o.tA;
+        rep->num_precond = o.nP;
+        rep->num_multiply = o.nM;
+    }
+    return st;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int psp_hip_create(const psp_hip_desc* desc, psp_hip_ctx** out)
+{
+    if (!out) return PSP_HIP_ERR_INVALID;
+    *out = nullptr;
+    if (!desc) return PSP_HIP_ERR_INVALID;
+    if (desc->precision != PSP_HIP_FRONTS_FP64 && desc->precision != PSP_HIP_FRONTS_FP32)
+        return PSP_HIP_ERR_INVALID;
+    psp_hip_ctx* c = new psp_hip_ctx();
+    c->desc = *desc;
+    std::string msg;
+    const int st = BuildPlan(*desc, c->plan, msg);
+    if (st != PSP_HIP_OK) {
+        delete c;
+        return st;
+    }
+    *out = c;
+    return PSP_HIP_OK;
+}
+
+int psp_hip_destroy(psp_hip_ctx* c)
+{
+    if (!c) return PSP_HIP_OK;
+    if (c->devReady) (void)hipStreamSynchronize(c->stream);
+    FreeWorkspace(c);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+    return PSP_HIP_OK;
+}
+
+const char* psp_hip_last_error(const psp_hip_ctx* c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+int psp_hip_set_stream(psp_hip_ctx* c, void* stream)
+{
+    if (!c) return PSP_HIP_ERR_INVALID;
+    c->stream = stream ? (hipStream_t)stream : c->own;
+    return PSP_HIP_OK;
+}
+
+int psp_hip_factor(psp_hip_ctx* c, const double* velocity, int scheme, psp_hip_factor_report* rep)
+{
+    if (!c) return PSP_HIP_ERR_INVALID;
+    if (c->desc.omega == 0.0)
+        return Fail(c, PSP_HIP_ERR_INVALID, "PML does not work at zero frequency");
+    if (scheme != PSP_HIP_LDL_1D && scheme != PSP_HIP_LDL_SELINV_1D)
+        return Fail(c, PSP_HIP_ERR_INVALID, "Invalid panelScheme value");
+    if (!velocity) return Fail(c, PSP_HIP_ERR_INVALID, "Velocity grid is incorrect");
+    const double t0 = Now();
+    int st = EnsureDevice(c);
+    if (st) return st;
+    const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
+    HIPCHK(c, hipMemcpyAsync(c->vel, velocity, sizeof(double) * N, hipMemcpyHostToDevice,
+                             c->stream));
+    double tf = 0, tl = 0, ti = 0;
+    st = c->desc.precision == PSP_HIP_FRONTS_FP32 ? FactorClasses<float>(c, &tf, &tl, &ti)
+                                                  : FactorClasses<double>(c, &tf, &tl, &ti);
+    if (st) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->factored = true;
+    if (rep) {
+        rep->seconds_total = Now() - t0;
+        rep->seconds_fill = tf;
+        rep->seconds_ldl = tl;
+        rep->seconds_inverse = ti;
+        double fl = 0, sc = 0, by = 0;
+        int nf = 0;
+        for (int cl = 0; cl < 3; ++cl) {
+            if (!c->plan.present[cl]) continue;
+            const double np = (double)c->classPanels[cl].size();
+            fl += np * c->plan.cls[cl].flops;
+            sc += np * c->plan.cls[cl].schurFlops;
+            by += np * c->plan.cls[cl].factorEntries * FrontBytes(c);
+            nf += (int)np * (int)c->plan.cls[cl].nodes.size();
+        }
+        rep->flops = fl;
+        rep->schur_flops = sc;
+        rep->bytes_factor = by;
+        rep->num_panels = c->plan.geo.numPanels;
+        rep->num_fronts = nf;
+    }
+    return PSP_HIP_OK;
+}
+
+int psp_hip_solve(psp_hip_ctx* c, void* B, int k, int m, double relTol, int maxIts, int flags,
+                  psp_hip_solve_report* rep)
+{
+    return RunSolve(c, B, k, m, relTol, maxIts, flags, 0, nullptr, rep);
+}
+
+int psp_hip_solve_fixed(psp_hip_ctx* c, void* B, int k, int m, int steps, int flags,
+                        double* relHist)
+{
+    if (steps < 1) return c ? Fail(c, PSP_HIP_ERR_INVALID, "steps must be >= 1") : 1;
+    return RunSolve(c, B, k, m, 0.0, 0, flags, steps, relHist, nullptr);
+}
+
+int psp_hip_multiply(psp_hip_ctx* c, const void* x, void* y, int k)
+{
+    int rc = CheckSolveArgs(c, x, k, 1);
+    if (rc) return rc;
+    LaunchSpmv(Stencil(c), c->vel, (const c128*)x, nullptr, (c128*)y, k, 0, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSP_HIP_OK;
+}
+
+int psp_hip_precondition(psp_hip_ctx* c, void* B, int k)
+{
+    int rc = CheckSolveArgs(c, B, k, 1);
+    if (rc) return rc;
+    if ((rc = EnsureWorkspace(c, k, std::max(1, c->wsK == k ? c->wsM : 1)))) return rc;
+    Precondition(c, (c128*)B, k);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSP_HIP_OK;
+}
+
+int psp_hip_solve_panel(psp_hip_ctx* c, void* B, int k, int panel)
+{
+    int rc = CheckSolveArgs(c, B, k, 1);
+    if (rc) return rc;
+    if (panel < 0 || panel >= c->plan.geo.numPanels)
+        return Fail(c, PSP_HIP_ERR_INVALID, "panel index out of range");
+    if ((rc = EnsureWorkspace(c, k, std::max(1, c->wsK == k ? c->wsM : 1)))) return rc;
+    if (c->desc.precision == PSP_HIP_FRONTS_FP32)
+        PanelForward<float>(c, (c128*)B, k, panel);
+    else
+        PanelForward<double>(c, (c128*)B, k, panel);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSP_HIP_OK;
+}
+
+int psp_hip_adjoint_gradient(psp_hip_ctx* c, const void* u, const void* lambda, int k,
+                             double* grad, int accumulate)
+{
+    int rc = CheckSolveArgs(c, u, k, 1);
+    if (rc) return rc;
+    if (!lambda || !grad) return Fail(c, PSP_HIP_ERR_INVALID, "null adjoint buffers");
+    LaunchAdjointGrad(Stencil(c), (const c128*)u, (const c128*)lambda, k, grad, accumulate,
+                      c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSP_HIP_OK;
+}
+
+int psp_hip_num_panels(const psp_hip_ctx* c) { return c ? c->plan.geo.numPanels : -1; }
+
+int psp_hip_panel_info(const psp_hip_ctx* c, int p, int* vPanel, int* depth, int* padding,
+                       int* cls)
+{
+    if (!c || p < 0 || p >= c->plan.geo.numPanels) return PSP_HIP_ERR_INVALID;
+    const Geometry& g = c->plan.geo;
+    if (vPanel) *vPanel = g.panelV(p);
+    if (depth) *depth = g.depth(p);
+    if (padding) *padding = g.padding(p);
+    if (cls) *cls = g.cls(p);
+    return PSP_HIP_OK;
+}
+
+int psp_hip_class_num_nodes(const psp_hip_ctx* c, int cls)
+{
+    if (!c || cls < 0 || cls > 2 || !c->plan.present[cls]) return 0;
+    return (int)c->plan.cls[cls].nodes.size();
+}
+
+int psp_hip_class_nodes(const psp_hip_ctx* c, int cls, int* off, int* size, int* parent,
+                        int* usize, int* structs)
+{
+    if (!c || cls < 0 || cls > 2 || !c->plan.present[cls]) return PSP_HIP_ERR_INVALID;
+    const ClassPlan& P = c->plan.cls[cls];
+    for (size_t t = 0; t < P.nodes.size(); ++t) {
+        if (off) off[t] = P.nodes[t].off;
+        if (size) size[t] = P.nodes[t].size;
+        if (parent) parent[t] = P.nodes[t].parent;
+        if (usize) usize[t] = P.nodes[t].usize;
+    }
+    if (structs) std::memcpy(structs, P.structs.data(), sizeof(int) * P.structs.size());
+    return PSP_HIP_OK;
+}
+
+int psp_hip_reordered_index(const psp_hip_ctx* c, int x, int y, int vLocal, int vSize)
+{
+    if (!c) return -1;
+    return NestedIndex(x, y, vLocal, c->plan.geo.nx, c->plan.geo.ny, vSize, c->plan.geo.cutoff);
+}
+
+int psp_hip_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,640 @@
+// Host symbolic analysis and static GPU work lists.  See symbolic.hpp.
+#include "symbolic.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <stack>
+
+namespace psp_amd {
+
+// --------------------------------------------------------------------------
+// Geometry: DistHelmholtz ctor panel split (main.hpp:35-57) and the panel
+// helpers PanelPadding/PanelDepth/PanelV (main.hpp:728-789).
+// --------------------------------------------------------------------------
+int Geometry::init(const psp_hip_desc& d)
+{
+    nx = d.nx;
+    ny = d.ny;
+    nz = d.nz;
+    bz = d.bz;
+    P = d.planesPerPanel;
+    cutoff = d.cutoff;
+    topPML = d.topBC == PSP_HIP_PML;
+    bottomDepth = bz + P;
+    topOrigDepth = topPML ? bz + P : P;
+    if (nz <= bottomDepth + topOrigDepth) return PSP_HIP_ERR_SHALLOW;
+    innerDepth = nz - (bottomDepth + topOrigDepth);
+    leftoverDepth = innerDepth % P;
+    haveLeftover = leftoverDepth != 0;
+    numFullInner = innerDepth / P;
+    numPanels = 1 + numFullInner + (haveLeftover ? 1 : 0) + 1;
+    return PSP_HIP_OK;
+}
+
+int Geometry::depth(int p) const
+{
+    if (p == 0) return bottomDepth;
+    if (p < 1 + numFullInner) return P;
+    if (haveLeftover && p == 1 + numFullInner) return leftoverDepth;
+    return topOrigDepth;
+}
+
+int Geometry::panelV(int p) const
+{
+    if (p == 0) return 0;
+    if (p < 1 + numFullInner) return bottomDepth + P * (p - 1);
+    if (haveLeftover && p == 1 + numFullInner) return bottomDepth + P * numFullInner;
+    return bottomDepth + P * numFullInner + leftoverDepth;
+}
+
+int Geometry::cls(int p) const
+{
+    if (p < 1 + numFullInner) return 0;
+    if (haveLeftover && p == 1 + numFullInner) return 1;
+    return 2;
+}
+
+int Geometry::classVSize(int c) const
+{
+    if (c == 0) return bottomDepth;  // == P + bz for the full inner panels
+    if (c == 1) return leftoverDepth + bz;
+    return topOrigDepth + bz;
+}
+
+// --------------------------------------------------------------------------
+// ReorderedIndexRecursion (main.hpp:1441-1477) with depthTilSerial = 0:
+// leaves are x-fastest boxes; otherwise the longer side is cut at
+// (len-1)/2 and the index is left part, right part, then the separator.
+// --------------------------------------------------------------------------
+int NestedIndex(int x, int y, int vl, int xs, int ys, int vs, int cutoff)
+{
+    if (xs * ys <= cutoff) return x + y * xs + vl * xs * ys;
+    if (xs >= ys) {
+        const int mid = (xs - 1) / 2;
+        if (x < mid) return NestedIndex(x, y, vl, mid, ys, vs, cutoff);
+        if (x == mid) return std::max(xs - 1, 0) * ys * vs + y + vl * ys;
+        return mid * ys * vs +
+               NestedIndex(x - mid - 1, y, vl, std::max(xs - mid - 1, 0), ys, vs, cutoff);
+    }
+    const int mid = (ys - 1) / 2;
+    if (y < mid) return NestedIndex(x, y, vl, xs, mid, vs, cutoff);
+    if (y == mid) return xs * std::max(ys - 1, 0) * vs + x + vl * xs;
+    return xs * mid * vs +
+           NestedIndex(x, y - mid - 1, vl, xs, std::max(ys - mid - 1, 0), vs, cutoff);
+}
+
+namespace {
+
+struct BoxRec {  // environment.hpp:117-121
+    int parent, nx, ny, xo, yo;
+    bool left;
+};
+
+int CountNodes(int bnx, int bny, int cutoff)  // NumLocalNodesRecursion
+{
+    if (bnx * bny <= cutoff) return 1;
+    if (bnx >= bny) {
+        const int mid = (bnx - 1) / 2;
+        return 1 + CountNodes(mid, bny, cutoff) +
+               CountNodes(std::max(bnx - mid - 1, 0), bny, cutoff);
+    }
+    const int mid = (bny - 1) / 2;
+    return 1 + CountNodes(bnx, mid, cutoff) +
+           CountNodes(bnx, std::max(bny - mid - 1, 0), cutoff);
+}
+
+int64_t CeilDiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Elimination tree of one panel class (FillPanelLocalElimTree,
+// main.hpp:1709-1900), built with the same box stack so node s runs from
+// the root (last) down; this yields a post-order numbering.
+void BuildTree(const Geometry& g, ClassPlan& C, std::vector<std::vector<int>>& orig)
+{
+    const int nx = g.nx, ny = g.ny, vs = C.vsize, cut = g.cutoff;
+    const int nn = CountNodes(nx, ny, cut);
+    C.nodes.assign(nn, Node());
+    orig.assign(nn, {});
+    auto nd = [&](int x, int y, int v) { return NestedIndex(x, y, v, nx, ny, vs, cut); };
+    std::stack<BoxRec> boxes;
+    boxes.push({-1, nx, ny, 0, 0, false});
+    std::vector<int> depthOf(nn, 0);
+    for (int s = nn - 1; s >= 0; --s) {
+        const BoxRec b = boxes.top();
+        boxes.pop();
+        Node& n = C.nodes[s];
+        n.parent = b.parent;
+        n.depth = b.parent < 0 ? 0 : C.nodes[b.parent].depth + 1;
+        if (b.parent >= 0) {
+            Node& p = C.nodes[b.parent];
+            p.child[b.left ? 0 : 1] = s;
+        }
+        std::vector<int>& os = orig[s];
+        if (b.nx * b.ny <= cut) {
+            n.size = b.nx * b.ny * vs;
+            n.off = n.size ? nd(b.xo, b.yo, 0) : 0;
+            if (b.xo > 0)
+                for (int i = 0; i < vs; ++i)
+                    for (int j = 0; j < b.ny; ++j) os.push_back(nd(b.xo - 1, b.yo + j, i));
+            if (b.xo + b.nx < nx)
+                for (int i = 0; i < vs; ++i)
+                    for (int j = 0; j < b.ny; ++j) os.push_back(nd(b.xo + b.nx, b.yo + j, i));
+            if (b.yo > 0)
+                for (int i = 0; i < vs; ++i)
+                    for (int j = 0; j < b.nx; ++j) os.push_back(nd(b.xo + j, b.yo - 1, i));
+            if (b.yo + b.ny < ny)
+                for (int i = 0; i < vs; ++i)
+                    for (int j = 0; j < b.nx; ++j) os.push_back(nd(b.xo + j, b.yo + b.ny, i));
+        } else if (b.nx >= b.ny) {
+            const int mid = (b.nx - 1) / 2;
+            n.size = b.ny * vs;
+            n.off = nd(b.xo + mid, b.yo, 0);
+            if (b.yo > 0)
+                for (int i = 0; i < vs; ++i) os.push_back(nd(b.xo + mid, b.yo - 1, i));
+            if (b.yo + b.ny < ny)
+                for (int i = 0; i < vs; ++i) os.push_back(nd(b.xo + mid, b.yo + b.ny, i));
+            boxes.push({s, mid, b.ny, b.xo, b.yo, true});
+            boxes.push({s, std::max(b.nx - mid - 1, 0), b.ny, b.xo + mid + 1, b.yo, false});
+        } else {
+            const int mid = (b.ny - 1) / 2;
+            n.size = b.nx * vs;
+            n.off = nd(b.xo, b.yo + mid, 0);
+            if (b.xo > 0)
+                for (int i = 0; i < vs; ++i) os.push_back(nd(b.xo - 1, b.yo + mid, i));
+            if (b.xo + b.nx < nx)
+                for (int i = 0; i < vs; ++i) os.push_back(nd(b.xo + b.nx, b.yo + mid, i));
+            boxes.push({s, b.nx, mid, b.xo, b.yo, true});
+            boxes.push({s, b.nx, std::max(b.ny - mid - 1, 0), b.xo, b.yo + mid + 1, false});
+        }
+        std::sort(os.begin(), os.end());
+    }
+    for (Node& n : C.nodes) n.nchild = (n.child[0] >= 0) + (n.child[1] >= 0);
+}
+
+// Symbolic factorization: struct(t) = (orig(t) U struct(children)) \ own(t).
+void Analyse(ClassPlan& C, const std::vector<std::vector<int>>& orig)
+{
+    const int nn = (int)C.nodes.size();
+    std::vector<std::vector<int>> st(nn);
+    for (int t = 0; t < nn; ++t) {
+        Node& n = C.nodes[t];
+        std::vector<int> acc = orig[t];
+        for (int k = 0; k < 2; ++k) {
+            if (n.child[k] < 0) continue;
+            std::vector<int> merged;
+            const std::vector<int>& cs = st[n.child[k]];
+            std::set_union(acc.begin(), acc.end(), cs.begin(), cs.end(),
+                           std::back_inserter(merged));
+            acc.swap(merged);
+        }
+        acc.erase(std::unique(acc.begin(), acc.end()), acc.end());
+        std::vector<int> out;
+        out.reserve(acc.size());
+        for (int gidx : acc)
+            if (gidx < n.off || gidx >= n.off + n.size) out.push_back(gidx);
+        st[t].swap(out);
+    }
+    int64_t total = 0;
+    for (int t = 0; t < nn; ++t) {
+        C.nodes[t].ustart = total;
+        C.nodes[t].usize = (int)st[t].size();
+        total += C.nodes[t].usize;
+    }
+    C.structs.resize(total);
+    C.relinds.assign(total, -1);
+    for (int t = 0; t < nn; ++t)
+        std::copy(st[t].begin(), st[t].end(), C.structs.begin() + C.nodes[t].ustart);
+    for (int t = 0; t < nn; ++t) {
+        const Node& n = C.nodes[t];
+        if (n.parent < 0) continue;
+        const Node& p = C.nodes[n.parent];
+        const int* ps = C.structs.data() + p.ustart;
+        for (int a = 0; a < n.usize; ++a) {
+            const int gidx = C.structs[n.ustart + a];
+            int rel;
+            if (gidx >= p.off && gidx < p.off + p.size) {
+                rel = gidx - p.off;
+            } else {
+                const int* f = std::lower_bound(ps, ps + p.usize, gidx);
+                rel = (f != ps + p.usize && *f == gidx) ? p.size + int(f - ps) : -1;
+            }
+            C.relinds[n.ustart + a] = rel;
+        }
+    }
+    C.maxDepth = 0;
+    for (const Node& n : C.nodes) C.maxDepth = std::max(C.maxDepth, n.depth);
+    C.levels.assign(C.maxDepth + 1, {});
+    for (int t = 0; t < nn; ++t) C.levels[C.nodes[t].depth].push_back(t);
+}
+
+// Front-assembly metadata: for every padded unknown its coordinates, node,
+// column, and the front rows of its strictly-lower couplings
+// (FormLowerColumnOfNode, initialize_finalize.hpp:589-806).
+void AssemblyMeta(const Geometry& g, ClassPlan& C)
+{
+    const int nx = g.nx, ny = g.ny, vs = C.vsize, cut = g.cutoff;
+    const int n = C.n;
+    C.qxy.assign(n, 0);
+    C.qv.assign(n, 0);
+    C.qnode.assign(n, -1);
+    C.qcol.assign(n, 0);
+    C.qrows.assign((size_t)n * 6, -1);
+    for (int v = 0; v < vs; ++v)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const int q = NestedIndex(x, y, v, nx, ny, vs, cut);
+                C.qxy[q] = x | (y << 16);
+                C.qv[q] = v;
+            }
+    for (int t = 0; t < (int)C.nodes.size(); ++t) {
+        const Node& nd = C.nodes[t];
+        for (int j = 0; j < nd.size; ++j) {
+            C.qnode[nd.off + j] = t;
+            C.qcol[nd.off + j] = j;
+        }
+    }
+    for (int q = 0; q < n; ++q) {
+        const int t = C.qnode[q];
+        const Node& nd = C.nodes[t];
+        const int x = C.qxy[q] & 0xffff, y = C.qxy[q] >> 16, v = C.qv[q];
+        const int nbx[6] = {x - 1, x + 1, x, x, x, x};
+        const int nby[6] = {y, y, y - 1, y + 1, y, y};
+        const int nbv[6] = {v, v, v, v, v - 1, v + 1};
+        for (int e = 0; e < 6; ++e) {
+            if (nbx[e] < 0 || nbx[e] >= nx || nby[e] < 0 || nby[e] >= ny || nbv[e] < 0 ||
+                nbv[e] >= vs)
+                continue;
+            const int ni = NestedIndex(nbx[e], nby[e], nbv[e], nx, ny, vs, cut);
+            if (ni <= q) continue;
+            int row;
+            if (ni < nd.off + nd.size) {
+                row = ni - nd.off;
+            } else {
+                const int* s0 = C.structs.data() + nd.ustart;
+                const int* f = std::lower_bound(s0, s0 + nd.usize, ni);
+                row = (f != s0 + nd.usize && *f == ni) ? nd.size + int(f - s0) : -1;
+            }
+            C.qrows[(size_t)q * 6 + e] = row;
+        }
+    }
+}
+
+void AddGemmTiles(ClassPlan& C, const GemmDesc& d, int tile)
+{
+    const int di = (int)C.gemms.size();
+    C.gemms.push_back(d);
+    for (int tn = 0; tn < d.N; tn += tile)
+        for (int tm = 0; tm < d.M; tm += tile) {
+            if ((d.flags & G_CLOWER) && tm + tile - 1 < tn) continue;
+            C.tiles.push_back({di, tm, tn, 0});
+        }
+}
+
+// Recursive combine steps of the unit-lower inverse; returns the height.
+struct Combine {
+    int lo, mid, hi;
+};
+int InvRecurse(int lo, int hi, std::vector<std::vector<Combine>>& byHeight)
+{
+    if (hi - lo <= 1) return 0;
+    const int mid = (lo + hi) / 2;
+    const int h = 1 + std::max(InvRecurse(lo, mid, byHeight), InvRecurse(mid, hi, byHeight));
+    if ((int)byHeight.size() <= h) byHeight.resize(h + 1);
+    byHeight[h].push_back({lo, mid, hi});
+    return h;
+}
+
+void Schedule(const Plan& plan, ClassPlan& C)
+{
+    const int nb = plan.blockNb, tile = plan.tile, rows = plan.solveRows;
+    const int nn = (int)C.nodes.size();
+    int64_t lo = 0, uo = 0, wo = 0;
+    for (Node& n : C.nodes) {
+        const int64_t fs = n.size + n.usize;
+        n.loff = lo;
+        lo += fs * n.size;
+        n.uoff = uo;
+        uo += (int64_t)n.usize * n.usize;
+        n.woff = wo;
+        wo += fs;
+    }
+    C.fstore = lo;
+    C.ustore = uo;
+    C.wstore = wo;
+
+    C.steps.assign(C.maxDepth + 1, {});
+    for (int d = C.maxDepth; d >= 0; --d) {
+        ClassPlan::LevelSteps& S = C.steps[d];
+        const std::vector<int>& lv = C.levels[d];
+        for (int slot = 0; slot < 2; ++slot) {
+            const int b0 = (int)C.extadd.size();
+            for (int t : lv) {
+                const Node& p = C.nodes[t];
+                const int c = p.child[slot];
+                if (c < 0) continue;
+                const Node& ch = C.nodes[c];
+                if (ch.usize == 0) continue;
+                for (int c0 = 0; c0 < ch.usize; c0 += 16) {
+                    ExtAddItem it;
+                    it.uChild = ch.uoff;
+                    it.lParent = p.loff;
+                    it.uParent = p.uoff;
+                    it.relStart = (int)ch.ustart;
+                    it.uc = ch.usize;
+                    it.sp = p.size;
+                    it.up = p.usize;
+                    it.col0 = c0;
+                    it.col1 = std::min(c0 + 16, ch.usize);
+                    C.extadd.push_back(it);
+                }
+            }
+            S.extadd[slot].push_back({b0, (int)C.extadd.size() - b0});
+        }
+        int maxs = 0;
+        for (int t : lv) maxs = std::max(maxs, C.nodes[t].size);
+        for (int j0 = 0; j0 < maxs; j0 += nb) {
+            const int l0 = (int)C.ldl.size();
+            for (int t : lv) {
+                const Node& n = C.nodes[t];
+                if (j0 >= n.size) continue;
+                LdlItem it;
+                it.l = n.loff;
+                it.ld = n.size + n.usize;
+                it.j0 = j0;
+                it.j1 = std::min(j0 + nb, n.size);
+                it.r0 = it.r1 = -1;
+                C.ldl.push_back(it);
+            }
+            S.ldlDiag.push_back({l0, (int)C.ldl.size() - l0});
+            const int l1 = (int)C.ldl.size();
+            for (int t : lv) {
+                const Node& n = C.nodes[t];
+                if (j0 >= n.size) continue;
+                const int j1 = std::min(j0 + nb, n.size);
+                const int fs = n.size + n.usize;
+                for (int r0 = j1; r0 < fs; r0 += 256) {
+                    LdlItem it;
+                    it.l = n.loff;
+                    it.ld = fs;
+                    it.j0 = j0;
+                    it.j1 = j1;
+                    it.r0 = r0;
+                    it.r1 = std::min(r0 + 256, fs);
+                    C.ldl.push_back(it);
+                }
+            }
+            S.ldlTrsm.push_back({l1, (int)C.ldl.size() - l1});
+            GemmLaunch gl;
+            gl.tileStart = (int)C.tiles.size();
+            for (int t : lv) {
+                const Node& n = C.nodes[t];
+                if (j0 >= n.size) continue;
+                const int j1 = std::min(j0 + nb, n.size);
+                if (j1 >= n.size) continue;
+                const int fs = n.size + n.usize;
+                GemmDesc g{};
+                g.a = n.loff + j1 + (int64_t)j0 * fs;
+                g.b = g.a;
+                g.c = n.loff + j1 + (int64_t)j1 * fs;
+                g.d = n.loff + j0 + (int64_t)j0 * fs;
+                g.lda = g.ldb = g.ldc = fs;
+                g.dinc = fs + 1;
+                g.M = fs - j1;
+                g.N = n.size - j1;
+                g.K = j1 - j0;
+                g.abuf = g.bbuf = g.cbuf = g.dbuf = BUF_L;
+                g.flags = G_BTRANS | G_HASD | G_NEG | G_CLOWER;
+                AddGemmTiles(C, g, tile);
+            }
+            gl.tileCount = (int)C.tiles.size() - gl.tileStart;
+            S.trail.push_back(gl);
+        }
+        S.schur.tileStart = (int)C.tiles.size();
+        for (int t : lv) {
+            const Node& n = C.nodes[t];
+            if (n.usize == 0 || n.size == 0 || n.parent < 0) continue;
+            const int fs = n.size + n.usize;
+            GemmDesc g{};
+            g.a = n.loff + n.size;
+            g.b = g.a;
+            g.c = n.uoff;
+            g.d = n.loff;
+            g.lda = g.ldb = fs;
+            g.ldc = n.usize;
+            g.dinc = fs + 1;
+            g.M = g.N = n.usize;
+            g.K = n.size;
+            g.abuf = g.bbuf = g.dbuf = BUF_L;
+            g.cbuf = BUF_U;
+            g.flags = G_BTRANS | G_HASD | G_NEG | G_CLOWER;
+            AddGemmTiles(C, g, tile);
+        }
+        S.schur.tileCount = (int)C.tiles.size() - S.schur.tileStart;
+    }
+
+    // Diagonal-block inversion: every nb block, then recursive combines.
+    std::vector<std::vector<std::pair<int, Combine>>> comb;  // by height
+    int64_t to = 0;
+    std::vector<std::vector<int64_t>> tOffsets(nn);
+    for (int t = 0; t < nn; ++t) {
+        Node& n = C.nodes[t];
+        n.toff = to;
+        if (n.size == 0) continue;
+        const int fs = n.size + n.usize;
+        const int nblk = (int)CeilDiv(n.size, nb);
+        for (int b = 0; b < nblk; ++b) {
+            InvItem it;
+            it.l = n.loff;
+            it.ld = fs;
+            it.j0 = b * nb;
+            it.j1 = std::min((b + 1) * nb, n.size);
+            it.pad_ = 0;
+            C.inv.push_back(it);
+        }
+        std::vector<std::vector<Combine>> byH;
+        InvRecurse(0, nblk, byH);
+        if (comb.size() < byH.size()) comb.resize(byH.size());
+        for (size_t h = 1; h < byH.size(); ++h)
+            for (const Combine& c : byH[h]) {
+                comb[h].push_back({t, c});
+                const int r0 = c.mid * nb, r1 = std::min(c.hi * nb, n.size);
+                const int c0 = c.lo * nb, c1 = c.mid * nb;
+                to += (int64_t)(r1 - r0) * (c1 - c0);
+            }
+    }
+    C.tstore = to;
+    // assign T offsets per combine in the same order
+    std::vector<int64_t> cursor(nn);
+    for (int t = 0; t < nn; ++t) cursor[t] = C.nodes[t].toff;
+    for (size_t h = 1; h < comb.size(); ++h) {
+        GemmLaunch gT, gX;
+        gT.tileStart = (int)C.tiles.size();
+        std::vector<GemmDesc> xs;
+        for (const auto& pc : comb[h]) {
+            const Node& n = C.nodes[pc.first];
+            const Combine& c = pc.second;
+            const int fs = n.size + n.usize;
+            const int r0 = c.mid * nb, r1 = std::min(c.hi * nb, n.size);
+            const int c0 = c.lo * nb, c1 = c.mid * nb;
+            const int64_t tOff = cursor[pc.first];
+            cursor[pc.first] += (int64_t)(r1 - r0) * (c1 - c0);
+            GemmDesc g{};  // T = C * Ainv
+            g.a = n.loff + r0 + (int64_t)c0 * fs;
+            g.lda = fs;
+            g.b = n.loff + c0 + (int64_t)c0 * fs;
+            g.ldb = fs;
+            g.c = tOff;
+            g.ldc = r1 - r0;
+            g.M = r1 - r0;
+            g.N = c1 - c0;
+            g.K = c1 - c0;
+            g.abuf = g.bbuf = BUF_L;
+            g.cbuf = BUF_T;
+            g.dbuf = BUF_L;
+            g.flags = G_BUNIT | G_BETA0;
+            AddGemmTiles(C, g, tile);
+            GemmDesc x{};  // X = -Binv * T
+            x.a = n.loff + r0 + (int64_t)r0 * fs;
+            x.lda = fs;
+            x.b = tOff;
+            x.ldb = r1 - r0;
+            x.c = n.loff + r0 + (int64_t)c0 * fs;
+            x.ldc = fs;
+            x.M = r1 - r0;
+            x.N = c1 - c0;
+            x.K = r1 - r0;
+            x.abuf = BUF_L;
+            x.bbuf = BUF_T;
+            x.cbuf = BUF_L;
+            x.dbuf = BUF_L;
+            x.flags = G_AUNIT | G_NEG | G_BETA0;
+            xs.push_back(x);
+        }
+        gT.tileCount = (int)C.tiles.size() - gT.tileStart;
+        gX.tileStart = (int)C.tiles.size();
+        for (const GemmDesc& x : xs) AddGemmTiles(C, x, tile);
+        gX.tileCount = (int)C.tiles.size() - gX.tileStart;
+        C.invGemm.push_back(gT);
+        C.invGemm.push_back(gX);
+    }
+
+    // Solve schedule.
+    C.solveLevels.assign(C.maxDepth + 1, {});
+    auto addItems = [&](int t, int count, int& b0, int& b1) {
+        const Node& n = C.nodes[t];
+        for (int r0 = 0; r0 < count; r0 += rows) {
+            SolveItem it;
+            it.l = n.loff;
+            it.w = n.woff;
+            it.off = n.off;
+            it.s = n.size;
+            it.u = n.usize;
+            it.structStart = (int)n.ustart;
+            it.r0 = r0;
+            it.r1 = std::min(r0 + rows, count);
+            it.child0 = n.child[0];
+            it.child1 = n.child[1];
+            C.solveItems.push_back(it);
+        }
+        b1 = (int)C.solveItems.size();
+        (void)b0;
+    };
+    for (int d = 0; d <= C.maxDepth; ++d) {
+        ClassPlan::SolveLevel& L = C.solveLevels[d];
+        const std::vector<int>& lv = C.levels[d];
+        L.gather0 = (int)C.solveItems.size();
+        L.gather1 = L.gather0;
+        for (int t : lv) {  // one item per node: init + children scatter
+            const Node& n = C.nodes[t];
+            SolveItem it;
+            it.l = n.loff;
+            it.w = n.woff;
+            it.off = n.off;
+            it.s = n.size;
+            it.u = n.usize;
+            it.structStart = (int)n.ustart;
+            it.r0 = 0;
+            it.r1 = n.size + n.usize;
+            it.child0 = n.child[0];
+            it.child1 = n.child[1];
+            C.solveItems.push_back(it);
+        }
+        L.gather1 = (int)C.solveItems.size();
+        L.fwd0 = L.fwd1 = (int)C.solveItems.size();
+        for (int t : lv) addItems(t, C.nodes[t].size, L.fwd0, L.fwd1);
+        L.fwd1 = (int)C.solveItems.size();
+        L.upd0 = L.upd1 = (int)C.solveItems.size();
+        for (int t : lv) addItems(t, C.nodes[t].usize, L.upd0, L.upd1);
+        L.upd1 = (int)C.solveItems.size();
+        L.bupd0 = L.bupd1 = (int)C.solveItems.size();
+        for (int t : lv) addItems(t, C.nodes[t].size, L.bupd0, L.bupd1);
+        L.bupd1 = (int)C.solveItems.size();
+        L.binv0 = L.binv1 = (int)C.solveItems.size();
+        for (int t : lv) addItems(t, C.nodes[t].size, L.binv0, L.binv1);
+        L.binv1 = (int)C.solveItems.size();
+    }
+
+    // Algorithmic counts (8 real flops per complex multiply-add).
+    double fl = 0, sc = 0, sv = 0, fe = 0;
+    for (const Node& n : C.nodes) {
+        const double s = n.size, u = n.usize;
+        fl += 8.0 * (s * s * s / 3.0 + u * s * s / 2.0 + u * (u + 1) * s / 2.0);
+        sc += 8.0 * (u * (u + 1) * s / 2.0);
+        sv += 8.0 * (s * s + 2.0 * u * s);
+        fe += (s + u) * s;
+    }
+    C.flops = fl;
+    C.schurFlops = sc;
+    C.solveFlopsPerRhs = sv;
+    C.factorEntries = fe;
+}
+
+}  // namespace
+
+int BuildPlan(const psp_hip_desc& d, Plan& plan, std::string& msg)
+{
+    if (d.nx < 1 || d.ny < 1 || d.nz < 1 || d.bz < 1 || d.planesPerPanel < 1 ||
+        d.cutoff < 1 || d.nx > 32767 || d.ny > 32767) {
+        msg = "invalid discretization sizes";
+        return PSP_HIP_ERR_INVALID;
+    }
+    if (!(d.omega == d.omega) || !(d.wx > 0) || !(d.wy > 0) || !(d.wz > 0)) {
+        msg = "invalid frequency or widths";
+        return PSP_HIP_ERR_INVALID;
+    }
+    const int st = plan.geo.init(d);
+    if (st != PSP_HIP_OK) {
+        msg = "The domain is very shallow. Please run a sparse-direct factorization instead.";
+        return st;
+    }
+    for (int c = 0; c < 3; ++c) {
+        if (c == 1 && !plan.geo.haveLeftover) continue;
+        ClassPlan& C = plan.cls[c];
+        C.cls = c;
+        C.vsize = plan.geo.classVSize(c);
+        C.n = plan.geo.nx * plan.geo.ny * C.vsize;
+        std::vector<std::vector<int>> orig;
+        BuildTree(plan.geo, C, orig);
+        Analyse(C, orig);
+        AssemblyMeta(plan.geo, C);
+        Schedule(plan, C);
+        // Every index the kernels dereference must land inside its front.
+        for (const Node& n : C.nodes)
+            for (int a = 0; a < n.usize; ++a) {
+                const int64_t e = n.ustart + a;
+                if (C.structs[e] < 0 || C.structs[e] >= C.n ||
+                    (n.parent >= 0 && C.relinds[e] < 0)) {
+                    msg = "internal error: inconsistent front structure";
+                    return PSP_HIP_ERR_INVALID;
+                }
+            }
+        for (int q = 0; q < C.n; ++q)
+            if (C.qnode[q] < 0) {
+                msg = "internal error: unknown without a front";
+                return PSP_HIP_ERR_INVALID;
+            }
+        plan.present[c] = true;
+    }
+    return PSP_HIP_OK;
+}
+
+}  // namespace psp_amd

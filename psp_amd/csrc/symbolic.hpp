@@ -1,0 +1,191 @@
+// Host-side symbolic analysis for the MI355X sweep solver: PSP's panel
+// split, analytic nested dissection, elimination tree and front structures,
+// plus the static device work lists (front assembly, blocked LDL^T steps,
+// Schur GEMMs, diagonal-block inversion, per-level solve phases).
+//
+// Everything here is integer work done once per DistHelmholtz object
+// (PSP's constructor, dist_helmholtz/main.hpp:12-436).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "psp_hip.h"
+
+namespace psp_amd {
+
+// ---------------------------------------------------------------------------
+// Panel geometry (dist_helmholtz/main.hpp:35-57, 728-821).
+// ---------------------------------------------------------------------------
+struct Geometry {
+    int nx = 0, ny = 0, nz = 0, bz = 0, P = 0, cutoff = 0;
+    bool topPML = true;
+    int bottomDepth = 0, topOrigDepth = 0, innerDepth = 0, leftoverDepth = 0;
+    int numFullInner = 0, numPanels = 0;
+    bool haveLeftover = false;
+
+    // Returns PSP_HIP_OK or PSP_HIP_ERR_SHALLOW.
+    int init(const psp_hip_desc& d);
+    int padding(int p) const { return p == 0 ? 0 : bz; }
+    int depth(int p) const;
+    int panelV(int p) const;
+    int cls(int p) const;  // 0 bottom/full inner, 1 leftover, 2 top
+    int vOffset(int p) const { return panelV(p) - padding(p); }
+    int classVSize(int c) const;
+};
+
+// In-panel nested-dissection index (ReorderedIndexRecursion,
+// dist_helmholtz/main.hpp:1425-1477, one process).
+int NestedIndex(int x, int y, int vLocal, int xSize, int ySize, int vSize,
+                int cutoff);
+
+// ---------------------------------------------------------------------------
+// Device work-list records.  All offsets are element offsets inside one
+// panel's slice of the named buffer; the kernels add batch * stride.
+// ---------------------------------------------------------------------------
+enum BufId : int16_t { BUF_L = 0, BUF_U = 1, BUF_T = 2 };
+
+enum GemmFlags : int32_t {
+    G_BTRANS = 1,   // B(k,n) = Bp[n + k*ldb]
+    G_AUNIT = 2,    // A is unit lower triangular (square)
+    G_BUNIT = 4,    // B is unit lower triangular (square)
+    G_CLOWER = 8,   // only C(i,j) with i >= j is touched
+    G_HASD = 16,    // multiply by diag(D) between A and B
+    G_NEG = 32,     // C -= ...
+    G_BETA0 = 64,   // C is overwritten instead of accumulated
+};
+
+struct GemmDesc {
+    int64_t a, b, c, d;
+    int32_t lda, ldb, ldc, dinc;
+    int32_t M, N, K;
+    int16_t abuf, bbuf, cbuf, dbuf;
+    int32_t flags;
+    int32_t pad_;
+};
+
+struct GemmTile {
+    int32_t desc;
+    int32_t tm, tn;
+    int32_t pad_;
+};
+
+// Extend-add of one child's Schur complement column range into its parent.
+struct ExtAddItem {
+    int64_t uChild;     // child's U offset (BUF_U)
+    int64_t lParent;    // parent's L offset (BUF_L)
+    int64_t uParent;    // parent's U offset (BUF_U)
+    int32_t relStart;   // into relinds
+    int32_t uc;         // child's update size
+    int32_t sp, up;     // parent's pivot and update sizes
+    int32_t col0, col1; // child's U column range handled by this item
+};
+
+// One block column step of the blocked LDL^T on one front: either the
+// factorization of the diagonal block [j0,j1)^2 (r0 == r1 == -1) or the
+// triangular solve of rows [r0, r1) against the factored diagonal block.
+struct LdlItem {
+    int64_t l;          // front's L offset
+    int32_t ld;         // s + u
+    int32_t j0, j1;     // block columns [j0, j1)
+    int32_t r0, r1;     // rows [r0, r1) below the diagonal block
+};
+
+// Inversion of one diagonal block of a unit lower triangular L_TL.
+struct InvItem {
+    int64_t l;
+    int32_t ld;
+    int32_t j0, j1;
+    int32_t pad_;
+};
+
+// Solve-phase item: one node (or a row block of a big node) of one panel.
+struct SolveItem {
+    int64_t l;          // L offset
+    int64_t w;          // node's work-vector offset (rows, times k)
+    int32_t off, s, u;  // node's first panel row, pivots, update size
+    int32_t structStart;
+    int32_t r0, r1;     // row range handled
+    int32_t child0, child1;  // node ids (-1 if none)
+};
+
+struct Node {
+    int off = 0, size = 0, usize = 0, parent = -1;
+    int child[2] = {-1, -1};
+    int nchild = 0;
+    int depth = 0;
+    int64_t ustart = 0;   // into structs / relinds
+    int64_t loff = 0;     // factor storage (BUF_L), (s+u) x s column-major
+    int64_t uoff = 0;     // Schur complement workspace (BUF_U), u x u
+    int64_t toff = 0;     // inversion temporary (BUF_T)
+    int64_t woff = 0;     // solve work vector rows (s+u)
+};
+
+struct GemmLaunch {
+    int32_t tileStart = 0, tileCount = 0;
+};
+
+struct ClassPlan {
+    int cls = 0;
+    int vsize = 0;
+    int n = 0;          // padded unknowns nx*ny*vsize
+    int maxDepth = 0;
+    std::vector<Node> nodes;            // post-order, root last
+    std::vector<int> structs;           // sorted lower structures
+    std::vector<int> relinds;           // row of each struct entry in parent
+    std::vector<std::vector<int>> levels;  // node ids per depth
+    int64_t fstore = 0, ustore = 0, tstore = 0, wstore = 0;
+
+    // Front assembly: per padded unknown q: packed x | y<<16, vLocal,
+    // node, column within node, and six front rows (x-1,x+1,y-1,y+1,v-1,v+1)
+    // or -1 when the coupling is absent or above the diagonal.
+    std::vector<int32_t> qxy, qv, qnode, qcol;
+    std::vector<int32_t> qrows;  // 6 per q
+
+    // Factorization schedule.
+    struct LevelSteps {
+        std::vector<std::pair<int, int>> extadd[2];  // item range per child slot
+        std::vector<std::pair<int, int>> ldlDiag;    // per block step: item range
+        std::vector<std::pair<int, int>> ldlTrsm;    // per block step: item range
+        std::vector<GemmLaunch> trail;               // per block step
+        GemmLaunch schur;
+    };
+    std::vector<ExtAddItem> extadd;
+    std::vector<LdlItem> ldl;
+    std::vector<GemmDesc> gemms;
+    std::vector<GemmTile> tiles;
+    std::vector<LevelSteps> steps;        // per depth (deepest first order)
+    std::vector<InvItem> inv;
+    std::vector<GemmLaunch> invGemm;      // alternating T = C*Ainv, X = -Binv*T
+
+    // Solve schedule: per depth, items for the forward gather, forward
+    // Linv, forward update, backward update, backward Linv^T.
+    std::vector<SolveItem> solveItems;
+    struct SolveLevel {
+        int gather0 = 0, gather1 = 0;
+        int fwd0 = 0, fwd1 = 0;
+        int upd0 = 0, upd1 = 0;
+        int bupd0 = 0, bupd1 = 0;
+        int binv0 = 0, binv1 = 0;
+    };
+    std::vector<SolveLevel> solveLevels;  // index = depth
+
+    // Algorithmic work per panel.
+    double flops = 0, schurFlops = 0, solveFlopsPerRhs = 0;
+    double factorEntries = 0;
+};
+
+struct Plan {
+    Geometry geo;
+    ClassPlan cls[3];
+    bool present[3] = {false, false, false};
+    int blockNb = 32;      // LDL^T block width
+    int tile = 64;         // GEMM tile
+    int solveRows = 256;   // rows per solve work item
+};
+
+// Builds everything; returns PSP_HIP_OK or an error code with `msg` set.
+int BuildPlan(const psp_hip_desc& desc, Plan& plan, std::string& msg);
+
+}  // namespace psp_amd

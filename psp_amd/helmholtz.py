@@ -1,0 +1,323 @@
+"""Python mirror of PSP's public solver API over the C ABI.
+
+Names, argument meanings, defaults and error behaviour follow
+include/psp/discretization.hpp, include/psp/dist_uniform_grid.hpp and
+include/psp/dist_helmholtz.hpp of the reference:
+
+    disc = Discretization(omega, nx, ny, nz, wx, wy, wz, PML, ..., pmlSize, sigma)
+    helm = DistHelmholtz(disc, damping=7.0, numPlanesPerPanel=4, cutoff=12)
+    helm.Initialize(velocity)            # DistUniformGrid of reals
+    helm.Solve(B, m=20, relTol=1e-4)     # DistUniformGrid of complex, in place
+    helm.Finalize()
+
+Right-hand sides may be host numpy arrays or torch complex128 CUDA tensors
+(HBM-resident); both use the DistUniformGrid layout at one process,
+(x + y*nx + z*nx*ny) * k + r.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (DIRICHLET, FRONTS_FP32, FRONTS_FP64, LDL_1D, LDL_SELINV_1D,
+                   PML, FactorReport, SolveReport)
+
+CLIQUE_LDL_1D = LDL_1D
+CLIQUE_LDL_SELINV_1D = LDL_SELINV_1D
+
+
+class LogicError(ValueError):
+    """psp::LogicError (std::logic_error)."""
+
+
+class RuntimeError_(RuntimeError):
+    """psp::RuntimeError (std::runtime_error)."""
+
+
+class BreakdownError(RuntimeError_):
+    """Exact Arnoldi breakdown; the reference returns silently here."""
+
+
+_LOGIC = {_lib.ERR_INVALID, _lib.ERR_SHALLOW, _lib.ERR_STATE}
+
+
+def _raise(code, ctx, what):
+    msg = ""
+    if ctx:
+        raw = _lib.lib().psp_hip_last_error(ctx)
+        msg = raw.decode() if raw else ""
+    text = f"{what}: {msg} (status {code})"
+    if code in _LOGIC:
+        raise LogicError(text)
+    if code == _lib.ERR_BREAKDOWN:
+        raise BreakdownError(text)
+    raise RuntimeError_(text)
+
+
+@dataclass
+class Discretization:
+    """Discretization<double> (include/psp/discretization.hpp:36-110)."""
+    omega: float
+    nx: int
+    ny: int
+    nz: int
+    wx: float
+    wy: float
+    wz: float
+    frontBC: int = PML
+    rightBC: int = PML
+    backBC: int = PML
+    leftBC: int = PML
+    topBC: int = PML
+    bx: int = 5
+    by: int = 5
+    bz: int = 5
+    sigmax: float | None = None
+    sigmay: float | None = None
+    sigmaz: float | None = None
+
+    def __post_init__(self):
+        # default sigma = 1.2 * max width (discretization.hpp:60-63)
+        s = 1.2 * max(self.wx, self.wy, self.wz)
+        self.sigmax = s if self.sigmax is None else self.sigmax
+        self.sigmay = s if self.sigmay is None else self.sigmay
+        self.sigmaz = s if self.sigmaz is None else self.sigmaz
+
+    @classmethod
+    def with_pml(cls, omega, nx, ny, nz, wx, wy, wz, front, right, back, left, top,
+                 pmlSize, sigma):
+        """The (..., pmlSize, sigma) constructor (discretization.hpp:96-109)."""
+        return cls(omega, nx, ny, nz, wx, wy, wz, front, right, back, left, top,
+                   pmlSize, pmlSize, pmlSize, sigma, sigma, sigma)
+
+
+class DistUniformGrid:
+    """DistUniformGrid<F> at one process (dist_uniform_grid.hpp:48-145).
+
+    Data layout: buffer[(x + y*nx + z*nx*ny) * numScalars + r].
+    """
+
+    def __init__(self, numScalars, nx, ny, nz, dtype=np.complex128, data=None):
+        self.nx, self.ny, self.nz, self.numScalars = nx, ny, nz, numScalars
+        if data is None:
+            data = np.zeros((nx * ny * nz, numScalars), dtype=dtype)
+        self.data = data
+
+    def XSize(self):
+        return self.nx
+
+    def YSize(self):
+        return self.ny
+
+    def ZSize(self):
+        return self.nz
+
+    def NumScalars(self):
+        return self.numScalars
+
+    def Buffer(self):
+        return self.data
+
+    def LocalIndex(self, x, y, z):
+        return (x + y * self.nx + z * self.nx * self.ny) * self.numScalars
+
+
+GridData = DistUniformGrid
+
+
+def _is_torch_cuda(a):
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+class DistHelmholtz:
+    """DistHelmholtz<double> on one MI355X (dist_helmholtz.hpp:21-283)."""
+
+    def __init__(self, disc: Discretization, damping=7.0, numPlanesPerPanel=4,
+                 cutoff=12, precision="fp64"):
+        prec = {"fp64": FRONTS_FP64, "fp32": FRONTS_FP32}[precision]
+        self.disc = disc
+        self.precision = precision
+        d = _lib.Desc(disc.omega, disc.nx, disc.ny, disc.nz, disc.wx, disc.wy, disc.wz,
+                      disc.frontBC, disc.rightBC, disc.backBC, disc.leftBC, disc.topBC,
+                      disc.bx, disc.by, disc.bz, disc.sigmax, disc.sigmay, disc.sigmaz,
+                      damping, numPlanesPerPanel, cutoff, prec)
+        self._desc = d
+        h = C.c_void_p()
+        st = _lib.lib().psp_hip_create(C.byref(d), C.byref(h))
+        if st != _lib.OK:
+            if st == _lib.ERR_SHALLOW:
+                raise LogicError("The domain is very shallow. Please run a sparse-direct "
+                                 "factorization instead.")
+            _raise(st, None, "DistHelmholtz")
+        self._h = h
+        self.N = disc.nx * disc.ny * disc.nz
+        self.factor_report = None
+
+    # ---- lifecycle -------------------------------------------------------
+    def __del__(self):
+        self.Finalize()
+
+    def Finalize(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().psp_hip_destroy(h)
+            self._h = None
+
+    def _ctx(self):
+        if not self._h:
+            raise LogicError("DistHelmholtz was finalized")
+        return self._h
+
+    def _bind_stream(self):
+        """Order the library's work after torch's pending work on the device:
+        run on torch's current stream, or, when that is the legacy default
+        stream, synchronize it and use the context's own stream."""
+        import torch
+        s = torch.cuda.current_stream().cuda_stream
+        if s == 0:
+            torch.cuda.current_stream().synchronize()
+        _lib.lib().psp_hip_set_stream(self._ctx(), C.c_void_p(s) if s else None)
+
+    # ---- Initialize ------------------------------------------------------
+    def Initialize(self, velocity, panelScheme=CLIQUE_LDL_SELINV_1D):
+        """Factor every panel (initialize_finalize.hpp:12-377)."""
+        v = velocity.data if isinstance(velocity, DistUniformGrid) else velocity
+        if isinstance(velocity, DistUniformGrid):
+            if (velocity.nx, velocity.ny, velocity.nz) != (self.disc.nx, self.disc.ny, self.disc.nz):
+                raise LogicError("Velocity grid is incorrect")
+            if velocity.numScalars != 1:
+                raise LogicError("Velocity grid should have one entry per point")
+        v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+        if v.size != self.N:
+            raise LogicError("Velocity grid is incorrect")
+        rep = FactorReport()
+        st = _lib.lib().psp_hip_factor(self._ctx(), v.ctypes.data_as(C.c_void_p), panelScheme,
+                                       C.byref(rep))
+        if st != _lib.OK:
+            _raise(st, self._h, "Initialize")
+        self.factor_report = rep
+        return rep
+
+    # ---- Solve -----------------------------------------------------------
+    def Solve(self, B, m=20, relTol=1e-4, maxIts=0, raise_on_error=True):
+        """Solve in place (solve.hpp:12-56); returns the SolveReport."""
+        arr = B.data if isinstance(B, DistUniformGrid) else B
+        rep = SolveReport()
+        if _is_torch_cuda(arr):
+            import torch
+            assert arr.dtype == torch.complex128 and arr.is_contiguous()
+            k = arr.shape[1] if arr.dim() == 2 else 1
+            self._bind_stream()
+            st = _lib.lib().psp_hip_solve(self._ctx(), C.c_void_p(arr.data_ptr()), k, m, relTol,
+                                          maxIts, _lib.DEVICE_POINTER, C.byref(rep))
+        else:
+            if not (isinstance(arr, np.ndarray) and arr.dtype == np.complex128
+                    and arr.flags.c_contiguous):
+                raise LogicError("B must be a C-contiguous complex128 array")
+            k = arr.shape[1] if arr.ndim == 2 else 1
+            st = _lib.lib().psp_hip_solve(self._ctx(), arr.ctypes.data_as(C.c_void_p), k, m,
+                                          relTol, maxIts, 0, C.byref(rep))
+        if st != _lib.OK and raise_on_error:
+            _raise(st, self._h, "Solve")
+        return rep
+
+    SolveWithGMRES = Solve
+
+    def SolveFixed(self, B, m, steps):
+        """GMRES with exactly `steps` inner steps; returns the residual history."""
+        hist = np.zeros(steps)
+        arr = B
+        if _is_torch_cuda(arr):
+            k = arr.shape[1]
+            self._bind_stream()
+            st = _lib.lib().psp_hip_solve_fixed(self._ctx(), C.c_void_p(arr.data_ptr()), k, m,
+                                                steps, _lib.DEVICE_POINTER,
+                                                hist.ctypes.data_as(C.c_void_p))
+        else:
+            k = arr.shape[1]
+            st = _lib.lib().psp_hip_solve_fixed(self._ctx(), arr.ctypes.data_as(C.c_void_p), k,
+                                                m, steps, 0, hist.ctypes.data_as(C.c_void_p))
+        if st != _lib.OK:
+            _raise(st, self._h, "SolveFixed")
+        return hist
+
+    # ---- device operators (torch complex128 CUDA tensors, grid layout) ---
+    def Multiply(self, x, y):
+        self._bind_stream()
+        st = _lib.lib().psp_hip_multiply(self._ctx(), C.c_void_p(x.data_ptr()),
+                                         C.c_void_p(y.data_ptr()), x.shape[1])
+        if st != _lib.OK:
+            _raise(st, self._h, "Multiply")
+
+    def Precondition(self, B):
+        self._bind_stream()
+        st = _lib.lib().psp_hip_precondition(self._ctx(), C.c_void_p(B.data_ptr()), B.shape[1])
+        if st != _lib.OK:
+            _raise(st, self._h, "Precondition")
+
+    def SolvePanel(self, B, panel):
+        self._bind_stream()
+        st = _lib.lib().psp_hip_solve_panel(self._ctx(), C.c_void_p(B.data_ptr()), B.shape[1],
+                                            panel)
+        if st != _lib.OK:
+            _raise(st, self._h, "SolvePanel")
+
+    def AdjointGradient(self, u, lam, grad, accumulate=False):
+        self._bind_stream()
+        st = _lib.lib().psp_hip_adjoint_gradient(self._ctx(), C.c_void_p(u.data_ptr()),
+                                                 C.c_void_p(lam.data_ptr()), u.shape[1],
+                                                 C.c_void_p(grad.data_ptr()), int(accumulate))
+        if st != _lib.OK:
+            _raise(st, self._h, "AdjointGradient")
+
+    # ---- symbolic queries (host only) ------------------------------------
+    @property
+    def num_panels(self):
+        return _lib.lib().psp_hip_num_panels(self._ctx())
+
+    def panel_info(self, p):
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        st = _lib.lib().psp_hip_panel_info(self._ctx(), p, C.byref(a), C.byref(b), C.byref(c),
+                                           C.byref(d))
+        if st != _lib.OK:
+            _raise(st, self._h, "panel_info")
+        return dict(v=a.value, depth=b.value, padding=c.value, cls=d.value)
+
+    def class_nodes(self, cls):
+        L = _lib.lib()
+        n = L.psp_hip_class_num_nodes(self._ctx(), cls)
+        off, size, parent, usize = (np.empty(n, np.int32) for _ in range(4))
+        L.psp_hip_class_nodes(self._ctx(), cls, off.ctypes.data_as(C.c_void_p),
+                              size.ctypes.data_as(C.c_void_p), parent.ctypes.data_as(C.c_void_p),
+                              usize.ctypes.data_as(C.c_void_p), None)
+        st = np.empty(int(usize.sum()), np.int32)
+        L.psp_hip_class_nodes(self._ctx(), cls, None, None, None, None,
+                              st.ctypes.data_as(C.c_void_p))
+        starts = np.concatenate([[0], np.cumsum(usize)])
+        structs = [st[starts[i]:starts[i + 1]] for i in range(n)]
+        return dict(off=off, size=size, parent=parent, usize=usize, structs=structs)
+
+    def reordered_index(self, x, y, vl, vs):
+        return _lib.lib().psp_hip_reordered_index(self._ctx(), x, y, vl, vs)
+
+
+def unit_cube_discretization(n, nz=None, ppw=10.0, cmin=1.0, pml=5, sigma=1.5, ny=None):
+    """Helper: grid spacing h = 1/(n+1) and omega from PSP's points-per-
+    wavelength definition (initialize_finalize.hpp:60-69):
+    omega = 2 pi cmin n / (ppw w) with w = 1."""
+    ny = n if ny is None else ny
+    nz = n if nz is None else nz
+    h = 1.0 / (max(n, ny, nz) + 1)
+    wx, wy, wz = (n + 1) * h, (ny + 1) * h, (nz + 1) * h
+    wmax = max(wx, wy, wz)
+    nmax = max(n, ny, nz)
+    omega = 2 * np.pi * cmin * nmax / (ppw * wmax)
+    return Discretization.with_pml(omega, n, ny, nz, wx, wy, wz, PML, PML, PML, PML, PML,
+                                   pml, sigma)

@@ -1,0 +1,159 @@
+"""Synthetic workloads: velocity models, sources and discretizations.
+
+Conventions follow the reference's UnitCube driver (tests/UnitCube.cpp):
+the unit cube has widths 1 and spacing h = 1/(n+1); coordinates are
+X = x/(n+1) (UnitCube.cpp:105).  Non-cubic grids keep h uniform with
+w_i = (n_i + 1) h.  omega comes from PSP's points-per-wavelength definition
+(initialize_finalize.hpp:60-69): PPW = n * lambda / w with lambda =
+2 pi c_min / omega, i.e. omega = 2 pi c_min n / (PPW w) on the longest axis.
+
+All randomness is seeded (numpy PCG64): seed = 20261202 + config index, with
+a separate stream per purpose.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .helmholtz import PML, Discretization
+
+SEED0 = 20261202
+
+
+def _rng(cfg, purpose):
+    return np.random.default_rng([SEED0 + cfg, purpose])
+
+
+def grid_coords(nx, ny, nz):
+    """Physical coordinates of grid points, arrays of shape (nz, ny, nx)."""
+    h = 1.0 / (max(nx, ny, nz) + 1)
+    X = np.arange(nx) * h
+    Y = np.arange(ny) * h
+    Z = np.arange(nz) * h
+    zz, yy, xx = np.meshgrid(Z, Y, X, indexing="ij")
+    return xx, yy, zz, h
+
+
+def discretization(nx, ny, nz, c_min, ppw, pml=5, sigma=1.5, top=PML):
+    h = 1.0 / (max(nx, ny, nz) + 1)
+    wx, wy, wz = (nx + 1) * h, (ny + 1) * h, (nz + 1) * h
+    nmax = max(nx, ny, nz)
+    wmax = max(wx, wy, wz)
+    omega = 2 * np.pi * c_min * nmax / (ppw * wmax)
+    return Discretization.with_pml(omega, nx, ny, nz, wx, wy, wz, PML, PML, PML, PML, top,
+                                   pml, sigma)
+
+
+def gaussian_sources(nx, ny, nz, centers):
+    """Gaussian point sources exp(-|X - X0|^2 / (2 (2h)^2)), grid layout
+    (N, k) with N in natural order x + y*nx + z*nx*ny."""
+    xx, yy, zz, h = grid_coords(nx, ny, nz)
+    s = 2.0 * h
+    cols = []
+    for (cx, cy, cz) in centers:
+        g = np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) / (2 * s * s))
+        cols.append(g.reshape(-1))
+    return np.ascontiguousarray(np.stack(cols, axis=1).astype(np.complex128))
+
+
+def bump_velocity(nx, ny, nz, cfg, nbumps=32, background=3.0, lo=1.5, hi=4.5):
+    """Smooth random velocity: background + sum of seeded Gaussian bumps
+    (amplitudes U[-1.5, 1.5], widths U[5, 20] cells), clipped to [lo, hi]."""
+    rng = _rng(cfg, 1)
+    xx, yy, zz, h = grid_coords(nx, ny, nz)
+    c = np.full(xx.shape, background)
+    for _ in range(nbumps):
+        a = rng.uniform(-1.5, 1.5)
+        w = rng.uniform(5, 20) * h
+        cx, cy, cz = rng.uniform(0.1, 0.9, 3) * np.array([nx, ny, nz]) * h
+        c += a * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) / (2 * w * w))
+    return np.clip(c, lo, hi).reshape(-1)
+
+
+def interior_points(nx, ny, nz, count, cfg, pml=5):
+    rng = _rng(cfg, 2)
+    h = 1.0 / (max(nx, ny, nz) + 1)
+    def pick(n):
+        lo = min(pml + 2, (n - 1) // 2)
+        return rng.integers(lo, max(lo + 1, n - lo))
+    pts = []
+    for _ in range(count):
+        ix, iy, iz = pick(nx), pick(ny), pick(nz)
+        pts.append((ix * h, iy * h, iz * h))
+    return pts
+
+
+def thrust_belt_velocity(nx, ny, nz, cfg, nlayers=12, nfaults=3):
+    """Layered thrust-belt model: velocity rising 2.0 -> 6.0 with depth (z)
+    through seeded dipping interfaces, offset by planar thrust faults, with
+    +-5% smooth noise."""
+    rng = _rng(cfg, 1)
+    xx, yy, zz, h = grid_coords(nx, ny, nz)
+    depth = zz / (nz * h)  # 0 at top, ~1 at bottom
+    X = xx / (nx * h)
+    Y = yy / (ny * h)
+    shift = np.zeros_like(depth)
+    for _ in range(nfaults):
+        strike = rng.uniform(0, np.pi)
+        nrm = np.array([np.cos(strike), np.sin(strike)])
+        off = rng.uniform(0.3, 0.7)
+        throw = rng.uniform(5, 15) / nz
+        side = (X * nrm[0] + Y * nrm[1] - off + 0.3 * depth) > 0
+        shift = shift + np.where(side, throw, 0.0)
+    d = depth - shift
+    ifaces = np.sort(rng.uniform(0.05, 0.95, nlayers))
+    layer = np.zeros_like(d)
+    for i, z0 in enumerate(ifaces):
+        dip = np.deg2rad(rng.uniform(0, 30))
+        strike = rng.uniform(0, 2 * np.pi)
+        plane = z0 + np.tan(dip) * 0.1 * ((X - 0.5) * np.cos(strike) + (Y - 0.5) * np.sin(strike))
+        layer = layer + (d > plane)
+    c = 2.0 + 4.0 * layer / nlayers
+    noise = np.zeros_like(c)
+    for _ in range(16):
+        cx, cy, cz = rng.uniform(0, 1, 3) * np.array([nx, ny, nz]) * h
+        w = rng.uniform(8, 30) * h
+        noise += rng.uniform(-1, 1) * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) / (2 * w * w))
+    noise /= max(1e-12, np.abs(noise).max())
+    return (c * (1 + 0.05 * noise)).reshape(-1)
+
+
+@dataclass
+class Workload:
+    name: str
+    disc: Discretization
+    velocity: np.ndarray
+    B: np.ndarray
+
+
+def config1():
+    """40^3, c = 1, 10 PPW, one Gaussian source at the centre."""
+    n = 40
+    disc = discretization(n, n, n, 1.0, 10.0)
+    vel = np.ones(n ** 3)
+    B = gaussian_sources(n, n, n, [(0.5, 0.5, 0.5)])
+    return Workload("unit-cube 40^3 c=1 10ppw 1rhs", disc, vel, B)
+
+
+def config2(n=100, k=8):
+    """100^3 Gaussian-bump velocity in [1.5, 4.5], 10 PPW, 8 sources."""
+    vel = bump_velocity(n, n, n, cfg=2)
+    disc = discretization(n, n, n, float(vel.min()), 10.0)
+    B = gaussian_sources(n, n, n, interior_points(n, n, n, k, cfg=2))
+    return Workload(f"bumps {n}^3 10ppw {k}rhs", disc, vel, B)
+
+
+def config3(nx=200, ny=200, nz=100, k=64):
+    """Thrust belt 200x200x100 (depth = z = sweep axis), 8 PPW at c_min=2,
+    64 sources on an 8x8 lattice of the plane z = bz + 2."""
+    vel = thrust_belt_velocity(nx, ny, nz, cfg=3)
+    disc = discretization(nx, ny, nz, 2.0, 8.0)
+    h = 1.0 / (max(nx, ny, nz) + 1)
+    side = int(round(np.sqrt(k)))
+    xs = np.linspace(0.15, 0.85, side) * nx * h
+    ys = np.linspace(0.15, 0.85, side) * ny * h
+    z0 = (5 + 2) * h  # the surface is z = 0 (PSP sweeps up from z = nz-1)
+    centers = [(x, y, z0) for y in ys for x in xs][:k]
+    B = gaussian_sources(nx, ny, nz, centers)
+    return Workload(f"thrust-belt {nx}x{ny}x{nz} 8ppw {k}rhs", disc, vel, B)
