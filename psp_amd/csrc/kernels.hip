@@ -4,6 +4,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace psp_amd {
 
@@ -403,221 +404,173 @@ __global__ __launch_bounds__(kThreads) void k_inv_diag(const InvItem* items, cx<
 
 // --------------------------------------------------------------------------
 // Front solves on the padded panel vector Xp (rows x k, interleaved).
+// Per node: forward  F1 W = [Xp_own; 0] + children's update vectors,
+//                    F2 y = Linv W_top            -> Xp_own,
+//                    F3 W_bot -= L_BL y;
+//           backward B1 W_top = D^-1 y - L_BL^T x_struct,
+//                    B2 x = Linv^T W_top          -> Xp_own.
+// The products run as split-K tiles; the last tile of a row block (an
+// atomic counter, reset by that tile) sums the partials in slot order.
 // --------------------------------------------------------------------------
+enum { PH_F2 = 0, PH_F3 = 1, PH_B1 = 2, PH_B2 = 3 };
 
-// Forward gather: W = [Xp_own; 0] + extend-add of the children's updates.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_gather(const SolveItem* items, int start,
-                                                          const int64_t* nodeW,
-                                                          const int32_t* nodeS,
-                                                          const int32_t* nodeU,
-                                                          const int32_t* nodeUs,
-                                                          const int32_t* relinds,
-                                                          const cx<T>* Xp, cx<T>* W, int k)
+__global__ __launch_bounds__(kThreads) void k_gather(const GatherItem* items, int start,
+                                                      const NodeDev* nodes, const int32_t* ginv,
+                                                      const cx<T>* Xp, cx<T>* W, int k)
 {
-    const SolveItem it = items[start + blockIdx.x];
-    cx<T>* Wt = W + it.w * k;
-    const int fs = it.s + it.u;
-    for (int e = threadIdx.x; e < fs * k; e += blockDim.x) {
-        const int row = e / k;
-        Wt[e] = row < it.s ? Xp[(int64_t)it.off * k + e] : czero<T>();
+    const GatherItem it = items[start + blockIdx.x];
+    const NodeDev nd = nodes[it.node];
+    const int n = (it.r1 - it.r0) * k;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int r = it.r0 + e / k, rr = e % k;
+        cx<T> v = r < nd.s ? Xp[(int64_t)(nd.off + r) * k + rr] : czero<T>();
+        const int64_t wr = nd.w + r;
+        const int g0 = ginv[2 * wr], g1 = ginv[2 * wr + 1];
+        if (g0 >= 0) v = v + W[(int64_t)g0 * k + rr];
+        if (g1 >= 0) v = v + W[(int64_t)g1 * k + rr];
+        W[wr * k + rr] = v;
     }
-    __syncthreads();
-    const int ch[2] = {it.child0, it.child1};
-    for (int c = 0; c < 2; ++c) {
-        if (ch[c] < 0) continue;
-        const int cs = nodeS[ch[c]], cu = nodeU[ch[c]];
-        const cx<T>* Wc = W + (nodeW[ch[c]] + cs) * k;
-        const int32_t* rel = relinds + nodeUs[ch[c]];
-        for (int e = threadIdx.x; e < cu * k; e += blockDim.x) {
-            const int a = e / k, r = e - a * k;
-            cx<T>* p = Wt + (int64_t)rel[a] * k + r;
-            *p = *p + Wc[e];
+}
+
+template <typename T, int PH>
+__device__ __forceinline__ void MvFinal(const NodeDev& nd, const cx<T>* Lf, cx<T>* Xp, cx<T>* Wt,
+                                        int row, int r, int k, cx<T> a)
+{
+    if (PH == PH_F2 || PH == PH_B2) {
+        Xp[(int64_t)(nd.off + row) * k + r] = a;
+    } else if (PH == PH_F3) {
+        cx<T>* p = Wt + (int64_t)(nd.s + row) * k + r;
+        *p = *p - a;
+    } else {
+        const int fs = nd.s + nd.u;
+        const cx<T> dinv = cinv(Lf[row + (int64_t)row * fs]);
+        Wt[(int64_t)row * k + r] = Xp[(int64_t)(nd.off + row) * k + r] * dinv - a;
+    }
+}
+
+template <typename T, int PH>
+__global__ __launch_bounds__(kThreads) void k_mv(const MvTile* tiles, int start,
+                                                  const NodeDev* nodes, const int32_t* structs,
+                                                  const cx<T>* L, const cx<T>* LT, cx<T>* Xp,
+                                                  cx<T>* W, cx<T>* part, int32_t* counters, int k)
+{
+    constexpr int KC = 8, R = 64, CC = 256;
+    const MvTile tl = tiles[start + blockIdx.x];
+    const NodeDev nd = nodes[tl.node];
+    const int s = nd.s, fs = nd.s + nd.u;
+    const cx<T>* Lf = L + nd.l;
+    const cx<T>* Lt = LT + nd.l;
+    cx<T>* Wt = W + nd.w * k;
+    __shared__ cx<T> vs[CC][KC];
+    __shared__ cx<T> red[3][R][KC];
+    __shared__ int lastFlag;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int row = tl.row0 + lane;
+    const bool rowOk = lane < tl.nrows;
+    const int ncol = tl.col1 - tl.col0;
+    const bool multi = tl.ntiles > 1;
+    for (int r0 = 0; r0 < k; r0 += KC) {
+        const int kc = min(KC, k - r0);
+        for (int e = tid; e < ncol * KC; e += kThreads) {
+            const int c = e / KC, r = e % KC;
+            cx<T> v = czero<T>();
+            if (r < kc) {
+                const int col = tl.col0 + c;
+                if (PH == PH_F2 || PH == PH_B2)
+                    v = Wt[(int64_t)col * k + r0 + r];
+                else if (PH == PH_F3)
+                    v = Xp[(int64_t)(nd.off + col) * k + r0 + r];
+                else
+                    v = Xp[(int64_t)structs[nd.ustart + col] * k + r0 + r];
+            }
+            vs[c][r] = v;
+        }
+        __syncthreads();
+        cx<T> acc[KC];
+#pragma unroll
+        for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
+        if (rowOk) {
+            for (int c = wave; c < ncol; c += 4) {
+                const int col = tl.col0 + c;
+                cx<T> m;
+                if (PH == PH_F2) {
+                    if (col > row) break;  // columns ascend: the rest are above
+                    m = col == row ? cx<T>{T(1), T(0)} : Lf[row + (int64_t)col * fs];
+                } else if (PH == PH_F3) {
+                    m = Lf[s + row + (int64_t)col * fs];
+                } else if (PH == PH_B1) {
+                    m = Lt[row + (int64_t)(s + col) * s];
+                } else {
+                    if (col < row) continue;
+                    m = col == row ? cx<T>{T(1), T(0)} : Lt[row + (int64_t)col * s];
+                }
+#pragma unroll
+                for (int r = 0; r < KC; ++r) acc[r] = cfma(m, vs[c][r], acc[r]);
+            }
+        }
+        if (wave > 0) {
+#pragma unroll
+            for (int r = 0; r < KC; ++r) red[wave - 1][lane][r] = acc[r];
+        }
+        __syncthreads();
+        if (wave == 0 && rowOk) {
+#pragma unroll
+            for (int w = 0; w < 3; ++w)
+#pragma unroll
+                for (int r = 0; r < KC; ++r) acc[r] = acc[r] + red[w][lane][r];
+#pragma unroll
+            for (int r = 0; r < KC; ++r) {
+                if (r >= kc) continue;
+                if (multi)
+                    part[((int64_t)(tl.part + tl.slot) * R + lane) * k + r0 + r] = acc[r];
+                else
+                    MvFinal<T, PH>(nd, Lf, Xp, Wt, row, r0 + r, k, acc[r]);
+            }
         }
         __syncthreads();
     }
-}
-
-// Forward: y = Linv W_top for rows [r0, r1); y goes to Xp_own.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_inv(const SolveItem* items, int start,
-                                                       const cx<T>* L, cx<T>* Xp, const cx<T>* W,
-                                                       int k)
-{
-    constexpr int KC = 8, CH = 32;
-    const SolveItem it = items[start + blockIdx.x];
-    const cx<T>* Lt = L + it.l;
-    const int fs = it.s + it.u;
-    const cx<T>* Wt = W + it.w * k;
-    __shared__ cx<T> ws[CH][KC];
-    const int i = it.r0 + threadIdx.x;
-    for (int r0 = 0; r0 < k; r0 += KC) {
-        const int kc = min(KC, k - r0);
-        cx<T> acc[KC];
-#pragma unroll
-        for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
-        for (int c0 = 0; c0 < it.r1; c0 += CH) {
-            __syncthreads();
-            for (int e = threadIdx.x; e < CH * KC; e += blockDim.x) {
-                const int c = c0 + e / KC, r = e % KC;
-                ws[e / KC][r] = (c < it.s && r < kc) ? Wt[(int64_t)c * k + r0 + r] : czero<T>();
-            }
-            __syncthreads();
-            if (i < it.r1) {
-                const int cend = min(CH, i + 1 - c0);
-                for (int cc = 0; cc < cend; ++cc) {
-                    const int c = c0 + cc;
-                    const cx<T> lv = c == i ? cx<T>{T(1), T(0)} : Lt[i + (int64_t)c * fs];
-#pragma unroll
-                    for (int r = 0; r < KC; ++r) acc[r] = cfma(lv, ws[cc][r], acc[r]);
-                }
-            }
-        }
-        if (i < it.r1) {
-#pragma unroll
-            for (int r = 0; r < KC; ++r)
-                if (r < kc) Xp[(int64_t)(it.off + i) * k + r0 + r] = acc[r];
-        }
+    if (!multi) return;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const int prev = atomicAdd(&counters[tl.group], 1);
+        lastFlag = prev == tl.ntiles - 1;
     }
-}
-
-// Forward: W_bot[a] -= L_BL[a,:] y for a in [r0, r1).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_fwd_upd(const SolveItem* items, int start,
-                                                       const cx<T>* L, const cx<T>* Xp, cx<T>* W,
-                                                       int k)
-{
-    constexpr int KC = 8, CH = 32;
-    const SolveItem it = items[start + blockIdx.x];
-    const cx<T>* Lt = L + it.l;
-    const int fs = it.s + it.u;
-    cx<T>* Wb = W + (it.w + it.s) * k;
-    __shared__ cx<T> ys[CH][KC];
-    const int a = it.r0 + threadIdx.x;
-    for (int r0 = 0; r0 < k; r0 += KC) {
-        const int kc = min(KC, k - r0);
-        cx<T> acc[KC];
-#pragma unroll
-        for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
-        for (int c0 = 0; c0 < it.s; c0 += CH) {
-            __syncthreads();
-            for (int e = threadIdx.x; e < CH * KC; e += blockDim.x) {
-                const int c = c0 + e / KC, r = e % KC;
-                ys[e / KC][r] =
-                    (c < it.s && r < kc) ? Xp[(int64_t)(it.off + c) * k + r0 + r] : czero<T>();
-            }
-            __syncthreads();
-            if (a < it.r1) {
-                const int cend = min(CH, it.s - c0);
-                for (int cc = 0; cc < cend; ++cc) {
-                    const cx<T> lv = Lt[it.s + a + (int64_t)(c0 + cc) * fs];
-#pragma unroll
-                    for (int r = 0; r < KC; ++r) acc[r] = cfma(lv, ys[cc][r], acc[r]);
-                }
-            }
-        }
-        if (a < it.r1) {
-#pragma unroll
-            for (int r = 0; r < KC; ++r)
-                if (r < kc) {
-                    cx<T>* p = Wb + (int64_t)a * k + r0 + r;
-                    *p = *p - acc[r];
-                }
-        }
+    __syncthreads();
+    if (!lastFlag) return;
+    __threadfence();
+    for (int e = tid; e < tl.nrows * k; e += kThreads) {
+        const int rl = e / k, r = e % k;
+        cx<T> a = czero<T>();
+        for (int sl = 0; sl < tl.ntiles; ++sl)
+            a = a + part[((int64_t)(tl.part + sl) * R + rl) * k + r];
+        MvFinal<T, PH>(nd, Lf, Xp, Wt, tl.row0 + rl, r, k, a);
     }
+    if (tid == 0) counters[tl.group] = 0;
 }
 
-// Wave-level sum of a complex value.
+// LT = L^T per front, through a 32x32 LDS tile.
 template <typename T>
-__device__ __forceinline__ cx<T> WaveSum(cx<T> v)
+__global__ __launch_bounds__(kThreads) void k_transpose(const TransItem* items,
+                                                         const NodeDev* nodes, const cx<T>* L,
+                                                         cx<T>* LT, int64_t fstore)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        v.r += __shfl_xor(v.r, o, 64);
-        v.i += __shfl_xor(v.i, o, 64);
+    const TransItem it = items[blockIdx.x];
+    const NodeDev nd = nodes[it.node];
+    const int s = nd.s, fs = nd.s + nd.u;
+    const cx<T>* Lf = L + (int64_t)blockIdx.y * fstore + nd.l;
+    cx<T>* Lt = LT + (int64_t)blockIdx.y * fstore + nd.l;
+    __shared__ cx<T> tile[32][33];
+    const int t32 = threadIdx.x & 31, q = threadIdx.x >> 5;
+    for (int jj = q; jj < 32; jj += 8) {
+        const int i = it.i0 + t32, j = it.j0 + jj;
+        if (i < fs && j < s) tile[jj][t32] = Lf[i + (int64_t)j * fs];
     }
-    return v;
-}
-
-// Backward: W_top[j] = y_j / d_j - sum_a L_BL[a][j] Xp[struct[a]], one wave
-// per column j.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_upd(const SolveItem* items, int start,
-                                                       const cx<T>* L, const int32_t* structs,
-                                                       const cx<T>* Xp, cx<T>* W, int k)
-{
-    constexpr int KC = 8;
-    const SolveItem it = items[start + blockIdx.x];
-    const cx<T>* Lt = L + it.l;
-    const int fs = it.s + it.u;
-    cx<T>* Wt = W + it.w * k;
-    const int32_t* st = structs + it.structStart;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = it.r0 + wave; j < it.r1; j += kThreads / 64) {
-        const cx<T>* col = Lt + (int64_t)j * fs;
-        const cx<T> dinv = cinv(col[j]);
-        for (int r0 = 0; r0 < k; r0 += KC) {
-            const int kc = min(KC, k - r0);
-            cx<T> acc[KC];
-#pragma unroll
-            for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
-            for (int a = lane; a < it.u; a += 64) {
-                const cx<T> lv = col[it.s + a];
-                const cx<T>* xs = Xp + (int64_t)st[a] * k + r0;
-#pragma unroll
-                for (int r = 0; r < KC; ++r)
-                    if (r < kc) acc[r] = cfma(lv, xs[r], acc[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < KC; ++r) acc[r] = WaveSum(acc[r]);
-            if (lane == 0) {
-#pragma unroll
-                for (int r = 0; r < KC; ++r)
-                    if (r < kc) {
-                        const cx<T> y = Xp[(int64_t)(it.off + j) * k + r0 + r];
-                        Wt[(int64_t)j * k + r0 + r] = y * dinv - acc[r];
-                    }
-            }
-        }
-    }
-}
-
-// Backward: Xp_own[j] = sum_{i>=j} Linv[i][j] W_top[i], one wave per column.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_bwd_inv(const SolveItem* items, int start,
-                                                       const cx<T>* L, cx<T>* Xp, const cx<T>* W,
-                                                       int k)
-{
-    constexpr int KC = 8;
-    const SolveItem it = items[start + blockIdx.x];
-    const cx<T>* Lt = L + it.l;
-    const int fs = it.s + it.u;
-    const cx<T>* Wt = W + it.w * k;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = it.r0 + wave; j < it.r1; j += kThreads / 64) {
-        const cx<T>* col = Lt + (int64_t)j * fs;
-        for (int r0 = 0; r0 < k; r0 += KC) {
-            const int kc = min(KC, k - r0);
-            cx<T> acc[KC];
-#pragma unroll
-            for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
-            for (int i = j + 1 + lane; i < it.s; i += 64) {
-                const cx<T> lv = col[i];
-                const cx<T>* ts = Wt + (int64_t)i * k + r0;
-#pragma unroll
-                for (int r = 0; r < KC; ++r)
-                    if (r < kc) acc[r] = cfma(lv, ts[r], acc[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < KC; ++r) acc[r] = WaveSum(acc[r]);
-            if (lane == 0) {
-#pragma unroll
-                for (int r = 0; r < KC; ++r)
-                    if (r < kc)
-                        Xp[(int64_t)(it.off + j) * k + r0 + r] =
-                            Wt[(int64_t)j * k + r0 + r] + acc[r];
-            }
-        }
+    __syncthreads();
+    for (int il = q; il < 32; il += 8) {
+        const int j = it.j0 + t32, i = it.i0 + il;
+        if (i < fs && j < s) Lt[j + (int64_t)i * s] = tile[t32][il];
     }
 }
 
@@ -791,13 +744,21 @@ __global__ __launch_bounds__(kThreads) void k_colnorm_part(const c128* X, int64_
     }
 }
 
-__global__ void k_colnorm_final(const double* part, int nblk, int k, double* out)
+// One workgroup per column: fixed-order tree sum of the block partials.
+__global__ __launch_bounds__(kThreads) void k_colnorm_final(const double* part, int nblk, int k,
+                                                             double* out)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= k) return;
+    __shared__ double sh[kThreads];
+    const int r = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * k + r];
-    out[r] = sqrt(s);
+    for (int b = threadIdx.x; b < nblk; b += kThreads) s += part[(int64_t)b * k + r];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[r] = sqrt(sh[0]);
 }
 
 __global__ __launch_bounds__(kThreads) void k_coldot_part(const c128* V, const c128* W,
@@ -829,13 +790,20 @@ __global__ __launch_bounds__(kThreads) void k_coldot_part(const c128* V, const c
     }
 }
 
-__global__ void k_coldot_final(const c128* part, int nblk, int k, c128* out)
+__global__ __launch_bounds__(kThreads) void k_coldot_final(const c128* part, int nblk, int k,
+                                                            c128* out)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= k) return;
+    __shared__ c128 sh[kThreads];
+    const int r = blockIdx.x;
     c128 s = {0.0, 0.0};
-    for (int b = 0; b < nblk; ++b) s = s + part[(int64_t)b * k + r];
-    out[r] = s;
+    for (int b = threadIdx.x; b < nblk; b += kThreads) s = s + part[(int64_t)b * k + r];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = kThreads / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[r] = sh[0];
 }
 
 __global__ __launch_bounds__(kThreads) void k_axpy_cols(const c128* alpha, int sign,
@@ -952,34 +920,41 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s)
 }
 
 template <typename T>
-void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, int k,
-                      hipStream_t s)
+void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv,
+                      const SolveScratch& sc, int k, hipStream_t s)
 {
     const cx<T>* L = (const cx<T>*)c.L + (int64_t)b * c.fstore;
+    const cx<T>* LT = (const cx<T>*)c.LT + (int64_t)b * c.fstore;
     cx<T>* Xp = (cx<T>*)Xpv;
     cx<T>* W = (cx<T>*)Wv;
+    cx<T>* part = (cx<T>*)sc.part;
+    auto mv = [&](auto ph, const ClassPlan::Range& r) {
+        if (r.count <= 0) return;
+        constexpr int PH = decltype(ph)::value;
+        hipLaunchKernelGGL((k_mv<T, PH>), dim3(r.count), dim3(kThreads), 0, s, c.mv, r.start,
+                           c.nodes, c.structs, L, LT, Xp, W, part, sc.counters, k);
+    };
     for (int d = plan.maxDepth; d >= 0; --d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
-        if (lv.gather1 > lv.gather0)
-            hipLaunchKernelGGL((k_fwd_gather<T>), dim3(lv.gather1 - lv.gather0), dim3(kThreads), 0,
-                               s, c.solve, lv.gather0, c.nodeW, c.nodeS, c.nodeU, c.nodeUs,
-                               c.relinds, Xp, W, k);
-        if (lv.fwd1 > lv.fwd0)
-            hipLaunchKernelGGL((k_fwd_inv<T>), dim3(lv.fwd1 - lv.fwd0), dim3(kThreads), 0, s,
-                               c.solve, lv.fwd0, L, Xp, W, k);
-        if (lv.upd1 > lv.upd0)
-            hipLaunchKernelGGL((k_fwd_upd<T>), dim3(lv.upd1 - lv.upd0), dim3(kThreads), 0, s,
-                               c.solve, lv.upd0, L, Xp, W, k);
+        if (lv.f1.count > 0)
+            hipLaunchKernelGGL((k_gather<T>), dim3(lv.f1.count), dim3(kThreads), 0, s, c.gather,
+                               lv.f1.start, c.nodes, c.ginv, Xp, W, k);
+        mv(std::integral_constant<int, PH_F2>(), lv.f2);
+        mv(std::integral_constant<int, PH_F3>(), lv.f3);
     }
     for (int d = 0; d <= plan.maxDepth; ++d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
-        if (lv.bupd1 > lv.bupd0)
-            hipLaunchKernelGGL((k_bwd_upd<T>), dim3(lv.bupd1 - lv.bupd0), dim3(kThreads), 0, s,
-                               c.solve, lv.bupd0, L, c.structs, Xp, W, k);
-        if (lv.binv1 > lv.binv0)
-            hipLaunchKernelGGL((k_bwd_inv<T>), dim3(lv.binv1 - lv.binv0), dim3(kThreads), 0, s,
-                               c.solve, lv.binv0, L, Xp, W, k);
+        mv(std::integral_constant<int, PH_B1>(), lv.b1);
+        mv(std::integral_constant<int, PH_B2>(), lv.b2);
     }
+}
+
+template <typename T>
+void LaunchTranspose(const ClassDev& c, hipStream_t s)
+{
+    if (c.ntrans <= 0) return;
+    hipLaunchKernelGGL((k_transpose<T>), dim3(c.ntrans, c.npanels), dim3(kThreads), 0, s,
+                       c.trans, c.nodes, (const cx<T>*)c.L, (cx<T>*)c.LT, c.fstore);
 }
 
 template <typename T>
@@ -1030,7 +1005,7 @@ void LaunchColNorm2(const c128* x, int64_t np, int k, double* part, double* out,
 {
     const int g = ReducePartials(np);
     hipLaunchKernelGGL(k_colnorm_part, dim3(g), dim3(kThreads), 0, s, x, np, k, part);
-    hipLaunchKernelGGL(k_colnorm_final, dim3((k + 63) / 64), dim3(64), 0, s, part, g, k, out);
+    hipLaunchKernelGGL(k_colnorm_final, dim3(k), dim3(kThreads), 0, s, part, g, k, out);
 }
 
 void LaunchColDot(const c128* v, const c128* w, int64_t np, int k, c128* part, c128* out,
@@ -1038,7 +1013,7 @@ void LaunchColDot(const c128* v, const c128* w, int64_t np, int k, c128* part, c
 {
     const int g = ReducePartials(np);
     hipLaunchKernelGGL(k_coldot_part, dim3(g), dim3(kThreads), 0, s, v, w, np, k, part);
-    hipLaunchKernelGGL(k_coldot_final, dim3((k + 63) / 64), dim3(64), 0, s, part, g, k, out);
+    hipLaunchKernelGGL(k_coldot_final, dim3(k), dim3(kThreads), 0, s, part, g, k, out);
 }
 
 void LaunchAxpyCols(const c128* alpha, int sign, const c128* v, c128* w, int64_t np, int k,
@@ -1086,8 +1061,9 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
     template void LaunchLdlTrsm<T>(const ClassDev&, int, int, int, hipStream_t);                  \
     template void LaunchGemm<T>(const ClassDev&, int, int, hipStream_t);                          \
     template void LaunchInvDiag<T>(const ClassDev&, int, hipStream_t);                            \
-    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, int,  \
-                                      hipStream_t);                                               \
+    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*,       \
+                                      const SolveScratch&, int, hipStream_t);                     \
+    template void LaunchTranspose<T>(const ClassDev&, hipStream_t);                               \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
                                   int, hipStream_t);                                              \
     template void LaunchCopyOut<T>(const ClassDev&, c128*, const void*, int, int, int, int, int,  \

@@ -27,6 +27,13 @@ struct StencilArgs {
     double omega;
 };
 
+// Per-node data the solve kernels read.
+struct NodeDev {
+    int64_t l, w;      // factor and work-vector offsets
+    int32_t off, s, u; // first panel row, pivots, update size
+    int32_t ustart;    // into structs
+};
+
 // Per-class device copy of the static plan.
 struct ClassDev {
     int n = 0, vsize = 0, nnodes = 0, npanels = 0;
@@ -41,13 +48,24 @@ struct ClassDev {
     GemmDesc* gemms = nullptr;
     GemmTile* tiles = nullptr;
     InvItem* inv = nullptr;
-    SolveItem* solve = nullptr;
-    int ninv = 0;
+    NodeDev* nodes = nullptr;
+    GatherItem* gather = nullptr;
+    MvTile* mv = nullptr;
+    TransItem* trans = nullptr;
+    int32_t* ginv = nullptr;
+    int ninv = 0, ntrans = 0;
     int32_t* panelVOff = nullptr;   // per batch entry: panel vOffset
     c128* s3art = nullptr;          // per batch entry: vsize + 2 profile values
     void* L = nullptr;              // npanels * fstore
     void* U = nullptr;              // npanels * ustore
     void* T = nullptr;              // npanels * tstore
+    void* LT = nullptr;             // npanels * fstore: per front L^T (s x (s+u), ld s)
+};
+
+// Solve scratch: split-K partials and self-resetting group counters.
+struct SolveScratch {
+    void* part = nullptr;   // maxSlots * 64 * k front elements
+    int32_t* counters = nullptr;  // maxGroups, all zero between launches
 };
 
 // ---- launch wrappers (kernels.hip) ----
@@ -71,7 +89,9 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s);
 // (n x k, k interleaved) with node work vectors W (wstore x k).
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xp, void* W,
-                      int k, hipStream_t s);
+                      const SolveScratch& sc, int k, hipStream_t s);
+template <typename T>
+void LaunchTranspose(const ClassDev& c, hipStream_t s);
 // Natural-order <-> padded panel vector.  mode 0: B = Xp, 1: B += Xp.
 template <typename T>
 void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int ny, int nz,

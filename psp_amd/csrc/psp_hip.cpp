@@ -51,6 +51,7 @@ struct psp_hip_ctx {
     c128 *V = nullptr, *x = nullptr, *x0 = nullptr, *w = nullptr, *b = nullptr;
     double *partD = nullptr, *normD = nullptr;
     c128 *partC = nullptr, *Hcol = nullptr, *ydev = nullptr;
+    SolveScratch scratch;
 };
 
 namespace {
@@ -230,13 +231,23 @@ int EnsureDevice(psp_hip_ctx* c)
             (st = Upload(c, &D.relinds, P.relinds)) || (st = Upload(c, &D.extadd, P.extadd)) ||
             (st = Upload(c, &D.ldl, P.ldl)) || (st = Upload(c, &D.gemms, P.gemms)) ||
             (st = Upload(c, &D.tiles, P.tiles)) || (st = Upload(c, &D.inv, P.inv)) ||
-            (st = Upload(c, &D.solve, P.solveItems)) || (st = Upload(c, &D.panelVOff, vOff)))
+            (st = Upload(c, &D.gather, P.gather)) || (st = Upload(c, &D.mv, P.mv)) ||
+            (st = Upload(c, &D.trans, P.trans)) || (st = Upload(c, &D.ginv, P.ginv)) ||
+            (st = Upload(c, &D.panelVOff, vOff)))
             return st;
+        std::vector<NodeDev> nd(D.nnodes);
+        for (int t = 0; t < D.nnodes; ++t) {
+            const Node& n = P.nodes[t];
+            nd[t] = NodeDev{n.loff, n.woff, n.off, n.size, n.usize, (int32_t)n.ustart};
+        }
+        if ((st = Upload(c, &D.nodes, nd))) return st;
+        D.ntrans = (int)P.trans.size();
         std::vector<c128> s3c = ToC128(s3a);
         if ((st = Upload(c, &D.s3art, s3c))) return st;
         if ((st = Alloc(c, &D.L, eb * (size_t)D.npanels * D.fstore, c->allocs))) return st;
         if ((st = Alloc(c, &D.U, eb * (size_t)D.npanels * D.ustore, c->allocs))) return st;
         if ((st = Alloc(c, &D.T, eb * (size_t)D.npanels * D.tstore, c->allocs))) return st;
+        if ((st = Alloc(c, &D.LT, eb * (size_t)D.npanels * D.fstore, c->allocs))) return st;
     }
     c->devReady = true;
     return PSP_HIP_OK;
@@ -286,6 +297,16 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
     c->Hcol = (c128*)p;
     if ((st = Alloc(c, &p, sizeof(c128) * (m + 1) * k, c->wsAllocs))) return st;
     c->ydev = (c128*)p;
+    int64_t groups = 1, slots = 1;
+    for (int cl = 0; cl < 3; ++cl)
+        if (c->plan.present[cl]) {
+            groups = std::max<int64_t>(groups, c->plan.cls[cl].maxGroups);
+            slots = std::max<int64_t>(slots, c->plan.cls[cl].maxSlots);
+        }
+    if ((st = Alloc(c, &c->scratch.part, eb * slots * c->plan.mvRows * k, c->wsAllocs))) return st;
+    if ((st = Alloc(c, &p, sizeof(int32_t) * groups, c->wsAllocs))) return st;
+    c->scratch.counters = (int32_t*)p;
+    HIPCHK(c, hipMemset(c->scratch.counters, 0, sizeof(int32_t) * groups));
     c->wsK = k;
     c->wsM = m;
     return PSP_HIP_OK;
@@ -323,6 +344,7 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv)
         HIPCHK(c, hipEventRecord(e[2], s));
         LaunchInvDiag<T>(D, nb, s);
         for (const GemmLaunch& gl : P.invGemm) LaunchGemm<T>(D, gl.tileStart, gl.tileCount, s);
+        LaunchTranspose<T>(D, s);
         HIPCHK(c, hipEventRecord(e[3], s));
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventSynchronize(e[3]));
@@ -352,7 +374,7 @@ void PanelForward(psp_hip_ctx* c, c128* B, int k, int p)
     const ClassDev& D = c->dev[cl];
     hipStream_t s = c->stream;
     LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->scratch, k, s);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s);
 }
 
@@ -370,7 +392,7 @@ void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p)
     const int v2 = g.panelV(p) + g.depth(p) - 1;
     const int vlLast = g.padding(p) + g.depth(p) - 1;
     LaunchSupdiagSet<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->scratch, k, s);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 1, s);
 }
 

@@ -305,7 +305,7 @@ int InvRecurse(int lo, int hi, std::vector<std::vector<Combine>>& byHeight)
 
 void Schedule(const Plan& plan, ClassPlan& C)
 {
-    const int nb = plan.blockNb, tile = plan.tile, rows = plan.solveRows;
+    const int nb = plan.blockNb, tile = plan.tile;
     const int nn = (int)C.nodes.size();
     int64_t lo = 0, uo = 0, wo = 0;
     for (Node& n : C.nodes) {
@@ -518,59 +518,105 @@ void Schedule(const Plan& plan, ClassPlan& C)
     }
 
     // Solve schedule.
-    C.solveLevels.assign(C.maxDepth + 1, {});
-    auto addItems = [&](int t, int count, int& b0, int& b1) {
-        const Node& n = C.nodes[t];
-        for (int r0 = 0; r0 < count; r0 += rows) {
-            SolveItem it;
-            it.l = n.loff;
-            it.w = n.woff;
-            it.off = n.off;
-            it.s = n.size;
-            it.u = n.usize;
-            it.structStart = (int)n.ustart;
-            it.r0 = r0;
-            it.r1 = std::min(r0 + rows, count);
-            it.child0 = n.child[0];
-            it.child1 = n.child[1];
-            C.solveItems.push_back(it);
+    {
+        const int R = plan.mvRows, CC = plan.mvCols;
+        C.ginv.assign((size_t)2 * C.wstore, -1);
+        for (const Node& n : C.nodes)
+            for (int k = 0; k < 2; ++k) {
+                if (n.child[k] < 0) continue;
+                const Node& ch = C.nodes[n.child[k]];
+                for (int a = 0; a < ch.usize; ++a) {
+                    const int r = C.relinds[ch.ustart + a];
+                    C.ginv[(size_t)2 * (n.woff + r) + k] = (int32_t)(ch.woff + ch.size + a);
+                }
+            }
+        C.solveLevels.assign(C.maxDepth + 1, {});
+        // Adds the tiles of one product: rows [0, nrows) in blocks of R; for
+        // a row block starting at r0 the inner range is [lo(r0), hi(r0)).
+        int groups = 0, slots = 0;
+        auto addProduct = [&](int t, int nrows, auto lo, auto hi) {
+            for (int r0 = 0; r0 < nrows; r0 += R) {
+                const int c0 = lo(r0), c1 = hi(r0);
+                const int nt = c1 > c0 ? (int)CeilDiv(c1 - c0, CC) : 1;
+                for (int s = 0; s < nt; ++s) {
+                    MvTile m;
+                    m.node = t;
+                    m.row0 = r0;
+                    m.nrows = std::min(R, nrows - r0);
+                    m.col0 = c0 + s * CC;
+                    m.col1 = std::min(c1, c0 + (s + 1) * CC);
+                    if (c1 <= c0) m.col0 = m.col1 = c0;
+                    m.slot = s;
+                    m.ntiles = nt;
+                    m.group = groups;
+                    m.part = slots;
+                    m.pad_ = 0;
+                    C.mv.push_back(m);
+                }
+                ++groups;
+                if (nt > 1) slots += nt;
+            }
+        };
+        auto beginRange = [&](ClassPlan::Range& r) {
+            r.start = (int)C.mv.size();
+            groups = slots = 0;
+        };
+        auto endRange = [&](ClassPlan::Range& r) {
+            r.count = (int)C.mv.size() - r.start;
+            C.maxGroups = std::max(C.maxGroups, groups);
+            C.maxSlots = std::max(C.maxSlots, slots);
+        };
+        for (int d = 0; d <= C.maxDepth; ++d) {
+            ClassPlan::SolveLevel& L = C.solveLevels[d];
+            const std::vector<int>& lv = C.levels[d];
+            L.f1.start = (int)C.gather.size();
+            for (int t : lv) {
+                const Node& n = C.nodes[t];
+                const int fs = n.size + n.usize;
+                for (int r0 = 0; r0 < fs; r0 += 256)
+                    C.gather.push_back({t, r0, std::min(fs, r0 + 256), 0});
+            }
+            L.f1.count = (int)C.gather.size() - L.f1.start;
+            beginRange(L.f2);  // y = Linv W_top, lower triangular
+            for (int t : lv)
+                if (C.nodes[t].size > 0) {
+                    const int s = C.nodes[t].size;
+                    addProduct(t, s, [](int) { return 0; },
+                               [&](int r0) { return std::min(s, r0 + R); });
+                }
+            endRange(L.f2);
+            beginRange(L.f3);  // W_bot -= L_BL y
+            for (int t : lv) {
+                const Node& n = C.nodes[t];
+                if (n.size > 0 && n.usize > 0)
+                    addProduct(t, n.usize, [](int) { return 0; },
+                               [&](int) { return n.size; });
+            }
+            endRange(L.f3);
+            beginRange(L.b1);  // W_top = D^-1 y - L_BL^T x_struct
+            for (int t : lv) {
+                const Node& n = C.nodes[t];
+                if (n.size > 0)
+                    addProduct(t, n.size, [](int) { return 0; },
+                               [&](int) { return n.usize; });
+            }
+            endRange(L.b1);
+            beginRange(L.b2);  // Xp_own = Linv^T W_top, upper triangular
+            for (int t : lv) {
+                const Node& n = C.nodes[t];
+                if (n.size > 0) {
+                    const int s = n.size;
+                    addProduct(t, s, [](int r0) { return r0; }, [&](int) { return s; });
+                }
+            }
+            endRange(L.b2);
         }
-        b1 = (int)C.solveItems.size();
-        (void)b0;
-    };
-    for (int d = 0; d <= C.maxDepth; ++d) {
-        ClassPlan::SolveLevel& L = C.solveLevels[d];
-        const std::vector<int>& lv = C.levels[d];
-        L.gather0 = (int)C.solveItems.size();
-        L.gather1 = L.gather0;
-        for (int t : lv) {  // one item per node: init + children scatter
+        for (int t = 0; t < nn; ++t) {
             const Node& n = C.nodes[t];
-            SolveItem it;
-            it.l = n.loff;
-            it.w = n.woff;
-            it.off = n.off;
-            it.s = n.size;
-            it.u = n.usize;
-            it.structStart = (int)n.ustart;
-            it.r0 = 0;
-            it.r1 = n.size + n.usize;
-            it.child0 = n.child[0];
-            it.child1 = n.child[1];
-            C.solveItems.push_back(it);
+            const int fs = n.size + n.usize;
+            for (int i0 = 0; i0 < fs; i0 += 32)
+                for (int j0 = 0; j0 < n.size; j0 += 32) C.trans.push_back({t, i0, j0, 0});
         }
-        L.gather1 = (int)C.solveItems.size();
-        L.fwd0 = L.fwd1 = (int)C.solveItems.size();
-        for (int t : lv) addItems(t, C.nodes[t].size, L.fwd0, L.fwd1);
-        L.fwd1 = (int)C.solveItems.size();
-        L.upd0 = L.upd1 = (int)C.solveItems.size();
-        for (int t : lv) addItems(t, C.nodes[t].usize, L.upd0, L.upd1);
-        L.upd1 = (int)C.solveItems.size();
-        L.bupd0 = L.bupd1 = (int)C.solveItems.size();
-        for (int t : lv) addItems(t, C.nodes[t].size, L.bupd0, L.bupd1);
-        L.bupd1 = (int)C.solveItems.size();
-        L.binv0 = L.binv1 = (int)C.solveItems.size();
-        for (int t : lv) addItems(t, C.nodes[t].size, L.binv0, L.binv1);
-        L.binv1 = (int)C.solveItems.size();
     }
 
     // Algorithmic counts (8 real flops per complex multiply-add).

@@ -100,14 +100,28 @@ struct InvItem {
     int32_t pad_;
 };
 
-// Solve-phase item: one node (or a row block of a big node) of one panel.
-struct SolveItem {
-    int64_t l;          // L offset
-    int64_t w;          // node's work-vector offset (rows, times k)
-    int32_t off, s, u;  // node's first panel row, pivots, update size
-    int32_t structStart;
-    int32_t r0, r1;     // row range handled
-    int32_t child0, child1;  // node ids (-1 if none)
+// Solve phase 1 (forward gather): rows [r0, r1) of node `node`'s front
+// work vector W = [Xp_own; 0] + children's update vectors.
+struct GatherItem {
+    int32_t node, r0, r1, pad_;
+};
+
+// One tile of a split-K matrix-vector product of one node: output rows
+// [row0, row0 + nrows) (nrows <= 64), inner indices [col0, col1).  The
+// tiles of one row block form a group; the last tile of a group to finish
+// sums the group's partials in slot order.
+struct MvTile {
+    int32_t node;
+    int32_t row0, nrows;
+    int32_t col0, col1;
+    int32_t slot, ntiles, group;
+    int32_t part;  // first partial slot of the group
+    int32_t pad_;
+};
+
+// 32x32 block of the front transpose LT = L^T (s x (s+u), ld s).
+struct TransItem {
+    int32_t node, i0, j0, pad_;
 };
 
 struct Node {
@@ -159,17 +173,21 @@ struct ClassPlan {
     std::vector<InvItem> inv;
     std::vector<GemmLaunch> invGemm;      // alternating T = C*Ainv, X = -Binv*T
 
-    // Solve schedule: per depth, items for the forward gather, forward
-    // Linv, forward update, backward update, backward Linv^T.
-    std::vector<SolveItem> solveItems;
+    // Solve schedule.  Per depth: forward gather (F1), forward Linv (F2),
+    // forward update (F3), backward update (B1), backward Linv^T (B2).
+    // ginv[2*r + c]: row of child c's work vector feeding front row r (or -1).
+    std::vector<int32_t> ginv;
+    std::vector<GatherItem> gather;
+    std::vector<MvTile> mv;
+    struct Range {
+        int start = 0, count = 0;
+    };
     struct SolveLevel {
-        int gather0 = 0, gather1 = 0;
-        int fwd0 = 0, fwd1 = 0;
-        int upd0 = 0, upd1 = 0;
-        int bupd0 = 0, bupd1 = 0;
-        int binv0 = 0, binv1 = 0;
+        Range f1, f2, f3, b1, b2;
     };
     std::vector<SolveLevel> solveLevels;  // index = depth
+    int maxGroups = 1, maxSlots = 1;
+    std::vector<TransItem> trans;
 
     // Algorithmic work per panel.
     double flops = 0, schurFlops = 0, solveFlopsPerRhs = 0;
@@ -182,7 +200,8 @@ struct Plan {
     bool present[3] = {false, false, false};
     int blockNb = 32;      // LDL^T block width
     int tile = 64;         // GEMM tile
-    int solveRows = 256;   // rows per solve work item
+    int mvRows = 64;       // output rows per solve tile
+    int mvCols = 256;      // inner indices per solve tile
 };
 
 // Builds everything; returns PSP_HIP_OK or an error code with `msg` set.
