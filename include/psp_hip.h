@@ -49,6 +49,12 @@ enum { PSP_HIP_FRONTS_FP64 = 0, PSP_HIP_FRONTS_FP32 = 1 };
 /* PanelScheme (include/psp/dist_helmholtz.hpp:15-19). */
 enum { PSP_HIP_LDL_1D = 0, PSP_HIP_LDL_SELINV_1D = 1 };
 
+/* Options for psp_hip_set_option. */
+enum {
+    PSP_HIP_OPT_SWEEP_GRAPH = 1 /* 1 (default): replay the sweep from a captured
+                                   hipGraph; 0: launch it kernel by kernel */
+};
+
 /* Flags for psp_hip_solve. */
 enum {
     PSP_HIP_DEVICE_POINTER = 1 /* B is a device pointer (HBM-resident)   */
@@ -111,6 +117,9 @@ const char* psp_hip_last_error(const psp_hip_ctx* ctx);
 /* Use `stream` (a hipStream_t, or NULL for the context's own stream) for all
  * later device work of this context. */
 int psp_hip_set_stream(psp_hip_ctx* ctx, void* stream);
+
+/* Set a context option (PSP_HIP_OPT_*); no reference counterpart. */
+int psp_hip_set_option(psp_hip_ctx* ctx, int option, int value);
 
 /* DistHelmholtz::Initialize (initialize_finalize.hpp:12-377): velocity is a
  * host array of nx*ny*nz doubles in natural order x + y*nx + z*nx*ny.

@@ -16,10 +16,12 @@ PML, DIRICHLET = 0, 1
 FRONTS_FP64, FRONTS_FP32 = 0, 1
 LDL_1D, LDL_SELINV_1D = 0, 1
 DEVICE_POINTER = 1
+OPT_SWEEP_GRAPH = 1
 
 # Every entry point include/psp_hip.h declares.
 EXPORTS = (
     "psp_hip_create", "psp_hip_destroy", "psp_hip_last_error", "psp_hip_set_stream",
+    "psp_hip_set_option",
     "psp_hip_factor", "psp_hip_solve", "psp_hip_multiply", "psp_hip_precondition",
     "psp_hip_solve_panel", "psp_hip_solve_fixed", "psp_hip_adjoint_gradient",
     "psp_hip_num_panels", "psp_hip_panel_info", "psp_hip_class_num_nodes",
@@ -83,6 +85,7 @@ def lib():
         "psp_hip_destroy": (I, [P]),
         "psp_hip_last_error": (C.c_char_p, [P]),
         "psp_hip_set_stream": (I, [P, P]),
+        "psp_hip_set_option": (I, [P, I, I]),
         "psp_hip_factor": (I, [P, P, I, C.POINTER(FactorReport)]),
         "psp_hip_solve": (I, [P, P, I, I, D, I, I, C.POINTER(SolveReport)]),
         "psp_hip_multiply": (I, [P, P, P, I]),

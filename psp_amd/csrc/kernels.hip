@@ -11,6 +11,7 @@ namespace psp_amd {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kMvCols = 256;  // inner indices per solve tile (must match the plan)
 
 // --------------------------------------------------------------------------
 // Complex arithmetic (interleaved re, im).
@@ -110,6 +111,27 @@ __device__ __forceinline__ void GlobalRow(const StencilArgs& st, double alpha, i
     const c128 k2 = {wa * wa, 0.0};
     Stencil(st.s1[x], st.s1[x + 1], st.s1[x + 2], st.s2[y], st.s2[y + 1], st.s2[y + 2],
             st.s3[v], st.s3[v + 1], st.s3[v + 2], k2, st.hx, st.hy, st.hz, out);
+}
+
+// The same row from per-axis tables: xTermL = (s2 s3) (1/s1(x-1) + 1/s1(x))
+// / (2 hx^2) etc., i.e. FormGlobalRow's terms with the divisions hoisted into
+// host tables (equal up to rounding).
+__device__ __forceinline__ void GlobalRowFast(const StencilArgs& st, double alpha, int x, int y,
+                                              int v, c128 out[7])
+{
+    const c128 s1M = st.s1[x + 1], s2M = st.s2[y + 1], s3M = st.s3[v + 1];
+    const c128 xTop = s2M * s3M, yTop = s1M * s3M, zTop = s1M * s2M;
+    const c128 xL = xTop * st.ax[2 * x], xR = xTop * st.ax[2 * x + 1];
+    const c128 yL = yTop * st.ay[2 * y], yR = yTop * st.ay[2 * y + 1];
+    const c128 vL = zTop * st.av[2 * v], vR = zTop * st.av[2 * v + 1];
+    const double wa = st.omega / alpha;
+    out[0] = (xL + xR + yL + yR + vL + vR) - (zTop * s3M) * (wa * wa);
+    out[1] = -xL;
+    out[2] = -xR;
+    out[3] = -yL;
+    out[4] = -yR;
+    out[5] = -vL;
+    out[6] = -vR;
 }
 
 // --------------------------------------------------------------------------
@@ -414,140 +436,138 @@ __global__ __launch_bounds__(kThreads) void k_inv_diag(const InvItem* items, cx<
 // --------------------------------------------------------------------------
 enum { PH_F2 = 0, PH_F3 = 1, PH_B1 = 2, PH_B2 = 3 };
 
+// Forward input of front row r: the panel right-hand side on the node's own
+// rows plus the children's update vectors (extend-add through ginv).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_gather(const GatherItem* items, int start,
-                                                      const NodeDev* nodes, const int32_t* ginv,
-                                                      const cx<T>* Xp, cx<T>* W, int k)
+__device__ __forceinline__ cx<T> Gathered(const NodeDev& nd, const int32_t* ginv, const cx<T>* Xp,
+                                          const cx<T>* W, int r, int rr, int k)
 {
-    const GatherItem it = items[start + blockIdx.x];
-    const NodeDev nd = nodes[it.node];
-    const int n = (it.r1 - it.r0) * k;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const int r = it.r0 + e / k, rr = e % k;
-        cx<T> v = r < nd.s ? Xp[(int64_t)(nd.off + r) * k + rr] : czero<T>();
-        const int64_t wr = nd.w + r;
-        const int g0 = ginv[2 * wr], g1 = ginv[2 * wr + 1];
-        if (g0 >= 0) v = v + W[(int64_t)g0 * k + rr];
-        if (g1 >= 0) v = v + W[(int64_t)g1 * k + rr];
-        W[wr * k + rr] = v;
-    }
+    cx<T> v = r < nd.s ? Xp[(int64_t)(nd.off + r) * k + rr] : czero<T>();
+    const int64_t wr = nd.w + r;
+    const int g0 = ginv[2 * wr], g1 = ginv[2 * wr + 1];
+    if (g0 >= 0) v = v + W[(int64_t)g0 * k + rr];
+    if (g1 >= 0) v = v + W[(int64_t)g1 * k + rr];
+    return v;
 }
 
+// F2: W_top = y; F3: W_bot = gathered - sum; B1: W_top = y/d - sum; B2: Xp = x.
 template <typename T, int PH>
-__device__ __forceinline__ void MvFinal(const NodeDev& nd, const cx<T>* Lf, cx<T>* Xp, cx<T>* Wt,
-                                        int row, int r, int k, cx<T> a)
+__device__ __forceinline__ void MvFinal(const NodeDev& nd, const int32_t* ginv, const cx<T>* Lf,
+                                        cx<T>* Xp, cx<T>* W, int row, int r, int k, cx<T> a)
 {
-    if (PH == PH_F2 || PH == PH_B2) {
-        Xp[(int64_t)(nd.off + row) * k + r] = a;
+    cx<T>* Wt = W + nd.w * k;
+    if (PH == PH_F2) {
+        Wt[(int64_t)row * k + r] = a;
     } else if (PH == PH_F3) {
-        cx<T>* p = Wt + (int64_t)(nd.s + row) * k + r;
-        *p = *p - a;
-    } else {
+        Wt[(int64_t)(nd.s + row) * k + r] = Gathered(nd, ginv, Xp, W, nd.s + row, r, k) - a;
+    } else if (PH == PH_B1) {
         const int fs = nd.s + nd.u;
         const cx<T> dinv = cinv(Lf[row + (int64_t)row * fs]);
-        Wt[(int64_t)row * k + r] = Xp[(int64_t)(nd.off + row) * k + r] * dinv - a;
+        cx<T>* p = Wt + (int64_t)row * k + r;
+        *p = *p * dinv - a;
+    } else {
+        Xp[(int64_t)(nd.off + row) * k + r] = a;
     }
 }
 
+constexpr int kMvThreads = 512;  // 8 waves split each row block's inner indices
+
+// One workgroup per (node, block of 64 output rows); the inner index range
+// streams through LDS in chunks of kMvCols and the 8 waves' partial sums
+// are combined by a tree reduction in LDS, so every output element is
+// summed in a fixed order.
 template <typename T, int PH>
-__global__ __launch_bounds__(kThreads) void k_mv(const MvTile* tiles, int start,
-                                                  const NodeDev* nodes, const int32_t* structs,
-                                                  const cx<T>* L, const cx<T>* LT, cx<T>* Xp,
-                                                  cx<T>* W, cx<T>* part, int32_t* counters, int k)
+__global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int start,
+                                                    const NodeDev* nodes, const int32_t* structs,
+                                                    const int32_t* ginv, const cx<T>* L,
+                                                    const cx<T>* LT, cx<T>* Xp, cx<T>* W, int k)
 {
-    constexpr int KC = 8, R = 64, CC = 256;
+    constexpr int KC = 8, R = 64, CC = kMvCols, NW = kMvThreads / 64, U = 16;
     const MvTile tl = tiles[start + blockIdx.x];
     const NodeDev nd = nodes[tl.node];
     const int s = nd.s, fs = nd.s + nd.u;
     const cx<T>* Lf = L + nd.l;
     const cx<T>* Lt = LT + nd.l;
-    cx<T>* Wt = W + nd.w * k;
-    __shared__ cx<T> vs[CC][KC];
-    __shared__ cx<T> red[3][R][KC];
-    __shared__ int lastFlag;
+    const cx<T>* Wt = W + nd.w * k;
+    __shared__ cx<T> vs[CC][KC];  // staged vector chunk; reused for the wave reduction
+    static_assert(CC * KC >= (NW / 2) * R * KC, "reduction scratch must fit in vs");
+    cx<T>(*red)[R][KC] = reinterpret_cast<cx<T>(*)[R][KC]>(&vs[0][0]);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int row = tl.row0 + lane;
     const bool rowOk = lane < tl.nrows;
-    const int ncol = tl.col1 - tl.col0;
-    const bool multi = tl.ntiles > 1;
     for (int r0 = 0; r0 < k; r0 += KC) {
         const int kc = min(KC, k - r0);
-        for (int e = tid; e < ncol * KC; e += kThreads) {
-            const int c = e / KC, r = e % KC;
-            cx<T> v = czero<T>();
-            if (r < kc) {
-                const int col = tl.col0 + c;
-                if (PH == PH_F2 || PH == PH_B2)
-                    v = Wt[(int64_t)col * k + r0 + r];
-                else if (PH == PH_F3)
-                    v = Xp[(int64_t)(nd.off + col) * k + r0 + r];
-                else
-                    v = Xp[(int64_t)structs[nd.ustart + col] * k + r0 + r];
-            }
-            vs[c][r] = v;
-        }
-        __syncthreads();
         cx<T> acc[KC];
 #pragma unroll
         for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
-        if (rowOk) {
-            for (int c = wave; c < ncol; c += 4) {
-                const int col = tl.col0 + c;
-                cx<T> m;
-                if (PH == PH_F2) {
-                    if (col > row) break;  // columns ascend: the rest are above
-                    m = col == row ? cx<T>{T(1), T(0)} : Lf[row + (int64_t)col * fs];
-                } else if (PH == PH_F3) {
-                    m = Lf[s + row + (int64_t)col * fs];
-                } else if (PH == PH_B1) {
-                    m = Lt[row + (int64_t)(s + col) * s];
-                } else {
-                    if (col < row) continue;
-                    m = col == row ? cx<T>{T(1), T(0)} : Lt[row + (int64_t)col * s];
+        for (int cb = tl.col0; cb < tl.col1; cb += CC) {
+            const int ncol = min(CC, tl.col1 - cb);
+            __syncthreads();
+            for (int e = tid; e < ncol * KC; e += kMvThreads) {
+                const int c = e / KC, r = e % KC;
+                cx<T> v = czero<T>();
+                if (r < kc) {
+                    const int col = cb + c;
+                    if (PH == PH_F2)
+                        v = Gathered(nd, ginv, Xp, W, col, r0 + r, k);
+                    else if (PH == PH_F3 || PH == PH_B2)
+                        v = Wt[(int64_t)col * k + r0 + r];
+                    else
+                        v = Xp[(int64_t)structs[nd.ustart + col] * k + r0 + r];
+                }
+                vs[c][r] = v;
+            }
+            __syncthreads();
+            const int nIter = ncol > wave ? (ncol - wave + NW - 1) / NW : 0;
+            for (int i0 = 0; i0 < nIter; i0 += U) {
+                cx<T> m[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int col = cb + wave + NW * (i0 + u);
+                    cx<T> v = czero<T>();
+                    if (rowOk && i0 + u < nIter) {
+                        if (PH == PH_F2)
+                            v = col < row ? Lf[row + (int64_t)col * fs]
+                                          : (col == row ? cx<T>{T(1), T(0)} : czero<T>());
+                        else if (PH == PH_F3)
+                            v = Lf[s + row + (int64_t)col * fs];
+                        else if (PH == PH_B1)
+                            v = Lt[row + (int64_t)(s + col) * s];
+                        else
+                            v = col > row ? Lt[row + (int64_t)col * s]
+                                          : (col == row ? cx<T>{T(1), T(0)} : czero<T>());
+                    }
+                    m[u] = v;
                 }
 #pragma unroll
-                for (int r = 0; r < KC; ++r) acc[r] = cfma(m, vs[c][r], acc[r]);
+                for (int u = 0; u < U; ++u) {
+                    if (i0 + u < nIter) {
+                        const int c = wave + NW * (i0 + u);
+#pragma unroll
+                        for (int r = 0; r < KC; ++r) acc[r] = cfma(m[u], vs[c][r], acc[r]);
+                    }
+                }
             }
         }
-        if (wave > 0) {
+        for (int h = NW / 2; h >= 1; h >>= 1) {
+            __syncthreads();
+            if (wave >= h && wave < 2 * h) {
 #pragma unroll
-            for (int r = 0; r < KC; ++r) red[wave - 1][lane][r] = acc[r];
+                for (int r = 0; r < KC; ++r) red[wave - h][lane][r] = acc[r];
+            }
+            __syncthreads();
+            if (wave < h) {
+#pragma unroll
+                for (int r = 0; r < KC; ++r) acc[r] = acc[r] + red[wave][lane][r];
+            }
         }
-        __syncthreads();
         if (wave == 0 && rowOk) {
 #pragma unroll
-            for (int w = 0; w < 3; ++w)
-#pragma unroll
-                for (int r = 0; r < KC; ++r) acc[r] = acc[r] + red[w][lane][r];
-#pragma unroll
-            for (int r = 0; r < KC; ++r) {
-                if (r >= kc) continue;
-                if (multi)
-                    part[((int64_t)(tl.part + tl.slot) * R + lane) * k + r0 + r] = acc[r];
-                else
-                    MvFinal<T, PH>(nd, Lf, Xp, Wt, row, r0 + r, k, acc[r]);
-            }
+            for (int r = 0; r < KC; ++r)
+                if (r < kc) MvFinal<T, PH>(nd, ginv, Lf, Xp, W, row, r0 + r, k, acc[r]);
         }
         __syncthreads();
     }
-    if (!multi) return;
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const int prev = atomicAdd(&counters[tl.group], 1);
-        lastFlag = prev == tl.ntiles - 1;
-    }
-    __syncthreads();
-    if (!lastFlag) return;
-    __threadfence();
-    for (int e = tid; e < tl.nrows * k; e += kThreads) {
-        const int rl = e / k, r = e % k;
-        cx<T> a = czero<T>();
-        for (int sl = 0; sl < tl.ntiles; ++sl)
-            a = a + part[((int64_t)(tl.part + sl) * R + rl) * k + r];
-        MvFinal<T, PH>(nd, Lf, Xp, Wt, tl.row0 + rl, r, k, a);
-    }
-    if (tid == 0) counters[tl.group] = 0;
 }
 
 // LT = L^T per front, through a 32x32 LDS tile.
@@ -634,7 +654,7 @@ __global__ __launch_bounds__(kThreads) void k_subdiag(StencilArgs st, const doub
     const int z = st.nz - 1 - v1;
     const int64_t nat = xy + (int64_t)z * plane;
     c128 c[7];
-    GlobalRow(st, vel[nat], x, y, v1, c);
+    GlobalRowFast(st, vel[nat], x, y, v1, c);
     // row (x,y,v1) couples to (x,y,v1-1) = natural z+1 with value c[5]
     B[nat * k + r] = B[nat * k + r] - c[5] * B[(nat + plane) * k + r];
 }
@@ -653,7 +673,7 @@ __global__ __launch_bounds__(kThreads) void k_supdiag_set(StencilArgs st, const 
     const int z = st.nz - 1 - v2;
     const int64_t nat = xy + (int64_t)z * plane;
     c128 c[7];
-    GlobalRow(st, vel[nat], x, y, v2, c);
+    GlobalRowFast(st, vel[nat], x, y, v2, c);
     // row (x,y,v2) couples to (x,y,v2+1) = natural z-1 with value c[6]
     const c128 val = -(c[6] * B[(nat - plane) * k + r]);
     const int q = nat2q[xy + (int64_t)vlLast * plane];
@@ -677,7 +697,7 @@ __global__ __launch_bounds__(kThreads) void k_spmv(StencilArgs st, const double*
     const int z = (int)(p / plane);
     const int v = st.nz - 1 - z;
     c128 c[7];
-    GlobalRow(st, vel[p], x, y, v, c);
+    GlobalRowFast(st, vel[p], x, y, v, c);
     c128 acc = X[e] * c[0];
     if (x > 0) acc = cfma(c[1], X[e - k], acc);
     if (x + 1 < st.nx) acc = cfma(c[2], X[e + k], acc);
@@ -920,25 +940,21 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s)
 }
 
 template <typename T>
-void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv,
-                      const SolveScratch& sc, int k, hipStream_t s)
+void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, int k,
+                      hipStream_t s)
 {
     const cx<T>* L = (const cx<T>*)c.L + (int64_t)b * c.fstore;
     const cx<T>* LT = (const cx<T>*)c.LT + (int64_t)b * c.fstore;
     cx<T>* Xp = (cx<T>*)Xpv;
     cx<T>* W = (cx<T>*)Wv;
-    cx<T>* part = (cx<T>*)sc.part;
     auto mv = [&](auto ph, const ClassPlan::Range& r) {
         if (r.count <= 0) return;
         constexpr int PH = decltype(ph)::value;
-        hipLaunchKernelGGL((k_mv<T, PH>), dim3(r.count), dim3(kThreads), 0, s, c.mv, r.start,
-                           c.nodes, c.structs, L, LT, Xp, W, part, sc.counters, k);
+        hipLaunchKernelGGL((k_mv<T, PH>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv, r.start,
+                           c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
     };
     for (int d = plan.maxDepth; d >= 0; --d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
-        if (lv.f1.count > 0)
-            hipLaunchKernelGGL((k_gather<T>), dim3(lv.f1.count), dim3(kThreads), 0, s, c.gather,
-                               lv.f1.start, c.nodes, c.ginv, Xp, W, k);
         mv(std::integral_constant<int, PH_F2>(), lv.f2);
         mv(std::integral_constant<int, PH_F3>(), lv.f3);
     }
@@ -1061,8 +1077,8 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
     template void LaunchLdlTrsm<T>(const ClassDev&, int, int, int, hipStream_t);                  \
     template void LaunchGemm<T>(const ClassDev&, int, int, hipStream_t);                          \
     template void LaunchInvDiag<T>(const ClassDev&, int, hipStream_t);                            \
-    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*,       \
-                                      const SolveScratch&, int, hipStream_t);                     \
+    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, int,  \
+                                      hipStream_t);                                               \
     template void LaunchTranspose<T>(const ClassDev&, hipStream_t);                               \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
                                   int, hipStream_t);                                              \

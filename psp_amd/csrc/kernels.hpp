@@ -22,6 +22,11 @@ struct StencilArgs {
     const c128* s1;   // nx + 2
     const c128* s2;   // ny + 2
     const c128* s3;   // nz + 2 (global operator)
+    // per-axis coupling factors (1/s(i-1) + 1/s(i)) / (2h^2) and
+    // (1/s(i+1) + 1/s(i)) / (2h^2), interleaved L, R per index
+    const c128* ax;   // 2 * nx
+    const c128* ay;   // 2 * ny
+    const c128* av;   // 2 * nz
     int nx, ny, nz;
     double hx, hy, hz;
     double omega;
@@ -49,7 +54,6 @@ struct ClassDev {
     GemmTile* tiles = nullptr;
     InvItem* inv = nullptr;
     NodeDev* nodes = nullptr;
-    GatherItem* gather = nullptr;
     MvTile* mv = nullptr;
     TransItem* trans = nullptr;
     int32_t* ginv = nullptr;
@@ -62,11 +66,6 @@ struct ClassDev {
     void* LT = nullptr;             // npanels * fstore: per front L^T (s x (s+u), ld s)
 };
 
-// Solve scratch: split-K partials and self-resetting group counters.
-struct SolveScratch {
-    void* part = nullptr;   // maxSlots * 64 * k front elements
-    int32_t* counters = nullptr;  // maxGroups, all zero between launches
-};
 
 // ---- launch wrappers (kernels.hip) ----
 void LaunchFillZero(void* p, int64_t bytes, hipStream_t s);
@@ -89,7 +88,7 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s);
 // (n x k, k interleaved) with node work vectors W (wstore x k).
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xp, void* W,
-                      const SolveScratch& sc, int k, hipStream_t s);
+                      int k, hipStream_t s);
 template <typename T>
 void LaunchTranspose(const ClassDev& c, hipStream_t s);
 // Natural-order <-> padded panel vector.  mode 0: B = Xp, 1: B += Xp.

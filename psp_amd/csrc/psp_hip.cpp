@@ -41,6 +41,7 @@ struct psp_hip_ctx {
     std::vector<void*> allocs;  // plan + factor storage
     double* vel = nullptr;
     c128 *s1 = nullptr, *s2 = nullptr, *s3 = nullptr;
+    c128 *ax = nullptr, *ay = nullptr, *av = nullptr;
     ClassDev dev[3];
     std::vector<int> panelBatch;   // panel -> batch index inside its class
     std::vector<std::vector<int>> classPanels;
@@ -51,7 +52,15 @@ struct psp_hip_ctx {
     c128 *V = nullptr, *x = nullptr, *x0 = nullptr, *w = nullptr, *b = nullptr;
     double *partD = nullptr, *normD = nullptr;
     c128 *partC = nullptr, *Hcol = nullptr, *ydev = nullptr;
-    SolveScratch scratch;
+    // captured sweeps, keyed on (block pointer, k)
+    bool useGraph = true;
+    struct SweepGraph {
+        void* B = nullptr;
+        int k = 0;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<SweepGraph> graphs;
 };
 
 namespace {
@@ -140,6 +149,9 @@ StencilArgs Stencil(const psp_hip_ctx* c)
     st.s1 = c->s1;
     st.s2 = c->s2;
     st.s3 = c->s3;
+    st.ax = c->ax;
+    st.ay = c->ay;
+    st.av = c->av;
     st.nx = c->desc.nx;
     st.ny = c->desc.ny;
     st.nz = c->desc.nz;
@@ -181,6 +193,18 @@ int EnsureDevice(psp_hip_ctx* c)
     if ((st = Upload(c, &c->s1, ToC128(s1)))) return st;
     if ((st = Upload(c, &c->s2, ToC128(s2)))) return st;
     if ((st = Upload(c, &c->s3, ToC128(s3)))) return st;
+    auto axis = [](const std::vector<zd>& sv, int n, double h) {
+        std::vector<zd> t(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            const zd rm = 1.0 / sv[i + 1];
+            t[2 * i] = (1.0 / sv[i] + rm) / (2 * h * h);
+            t[2 * i + 1] = (1.0 / sv[i + 2] + rm) / (2 * h * h);
+        }
+        return t;
+    };
+    if ((st = Upload(c, &c->ax, ToC128(axis(s1, d.nx, hx))))) return st;
+    if ((st = Upload(c, &c->ay, ToC128(axis(s2, d.ny, hy))))) return st;
+    if ((st = Upload(c, &c->av, ToC128(axis(s3, d.nz, hz))))) return st;
     void* vp = nullptr;
     if ((st = Alloc(c, &vp, sizeof(double) * (size_t)d.nx * d.ny * d.nz, c->allocs))) return st;
     c->vel = (double*)vp;
@@ -231,7 +255,7 @@ int EnsureDevice(psp_hip_ctx* c)
             (st = Upload(c, &D.relinds, P.relinds)) || (st = Upload(c, &D.extadd, P.extadd)) ||
             (st = Upload(c, &D.ldl, P.ldl)) || (st = Upload(c, &D.gemms, P.gemms)) ||
             (st = Upload(c, &D.tiles, P.tiles)) || (st = Upload(c, &D.inv, P.inv)) ||
-            (st = Upload(c, &D.gather, P.gather)) || (st = Upload(c, &D.mv, P.mv)) ||
+            (st = Upload(c, &D.mv, P.mv)) ||
             (st = Upload(c, &D.trans, P.trans)) || (st = Upload(c, &D.ginv, P.ginv)) ||
             (st = Upload(c, &D.panelVOff, vOff)))
             return st;
@@ -253,8 +277,18 @@ int EnsureDevice(psp_hip_ctx* c)
     return PSP_HIP_OK;
 }
 
+void FreeGraphs(psp_hip_ctx* c)
+{
+    for (auto& g : c->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    c->graphs.clear();
+}
+
 void FreeWorkspace(psp_hip_ctx* c)
 {
+    FreeGraphs(c);
     for (void* p : c->wsAllocs) (void)hipFree(p);
     c->wsAllocs.clear();
     c->wsK = c->wsM = 0;
@@ -297,16 +331,6 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
     c->Hcol = (c128*)p;
     if ((st = Alloc(c, &p, sizeof(c128) * (m + 1) * k, c->wsAllocs))) return st;
     c->ydev = (c128*)p;
-    int64_t groups = 1, slots = 1;
-    for (int cl = 0; cl < 3; ++cl)
-        if (c->plan.present[cl]) {
-            groups = std::max<int64_t>(groups, c->plan.cls[cl].maxGroups);
-            slots = std::max<int64_t>(slots, c->plan.cls[cl].maxSlots);
-        }
-    if ((st = Alloc(c, &c->scratch.part, eb * slots * c->plan.mvRows * k, c->wsAllocs))) return st;
-    if ((st = Alloc(c, &p, sizeof(int32_t) * groups, c->wsAllocs))) return st;
-    c->scratch.counters = (int32_t*)p;
-    HIPCHK(c, hipMemset(c->scratch.counters, 0, sizeof(int32_t) * groups));
     c->wsK = k;
     c->wsM = m;
     return PSP_HIP_OK;
@@ -374,7 +398,7 @@ void PanelForward(psp_hip_ctx* c, c128* B, int k, int p)
     const ClassDev& D = c->dev[cl];
     hipStream_t s = c->stream;
     LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->scratch, k, s);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s);
 }
 
@@ -392,7 +416,7 @@ void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p)
     const int v2 = g.panelV(p) + g.depth(p) - 1;
     const int vlLast = g.padding(p) + g.depth(p) - 1;
     LaunchSupdiagSet<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->scratch, k, s);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 1, s);
 }
 
@@ -409,12 +433,37 @@ void Sweep(psp_hip_ctx* c, c128* B, int k)
     for (int i = g.numPanels - 2; i >= 0; --i) PanelBackward<T>(c, B, k, i);
 }
 
-void Precondition(psp_hip_ctx* c, c128* B, int k)
+void SweepEager(psp_hip_ctx* c, c128* B, int k)
 {
     if (c->desc.precision == PSP_HIP_FRONTS_FP32)
         Sweep<float>(c, B, k);
     else
         Sweep<double>(c, B, k);
+}
+
+// Precondition: replays a hipGraph of the whole sweep (captured on first use
+// for this block and k) or launches it kernel by kernel.
+int Precondition(psp_hip_ctx* c, c128* B, int k)
+{
+    if (!c->useGraph || c->stream == nullptr) {
+        SweepEager(c, B, k);
+        return PSP_HIP_OK;
+    }
+    for (auto& g : c->graphs)
+        if (g.B == B && g.k == k) {
+            HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+            return PSP_HIP_OK;
+        }
+    psp_hip_ctx::SweepGraph g;
+    g.B = B;
+    g.k = k;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    SweepEager(c, B, k);
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g.graph));
+    HIPCHK(c, hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    c->graphs.push_back(g);
+    HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+    return PSP_HIP_OK;
 }
 
 int Norms(psp_hip_ctx* c, const c128* X, int k, std::vector<double>& out)
@@ -492,7 +541,7 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
     while (!converged) {
         HIPCHK(c, hipMemcpyAsync(c->x0, c->x, sizeof(c128) * n, hipMemcpyDeviceToDevice, s));
         double t0 = Now();
-        Precondition(c, c->w, k);
+        if ((rc = Precondition(c, c->w, k))) return rc;
         if ((rc = Norms(c, c->w, k, beta))) return rc;
         o.tP += Now() - t0;
         ++o.nP;
@@ -508,7 +557,7 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
             o.tM += Now() - t0;
             ++o.nM;
             t0 = Now();
-            Precondition(c, c->w, k);
+            if ((rc = Precondition(c, c->w, k))) return rc;
             HIPCHK(c, hipStreamSynchronize(s));
             o.tP += Now() - t0;
             ++o.nP;
@@ -685,8 +734,22 @@ const char* psp_hip_last_error(const psp_hip_ctx* c)
 int psp_hip_set_stream(psp_hip_ctx* c, void* stream)
 {
     if (!c) return PSP_HIP_ERR_INVALID;
-    c->stream = stream ? (hipStream_t)stream : c->own;
+    hipStream_t ns = stream ? (hipStream_t)stream : c->own;
+    if (ns != c->stream) FreeGraphs(c);  // graphs are launched on the capture stream's device
+    c->stream = ns;
     return PSP_HIP_OK;
+}
+
+int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
+{
+    if (!c) return PSP_HIP_ERR_INVALID;
+    if (option == PSP_HIP_OPT_SWEEP_GRAPH) {
+        if (c->devReady) (void)hipStreamSynchronize(c->stream);
+        FreeGraphs(c);
+        c->useGraph = value != 0;
+        return PSP_HIP_OK;
+    }
+    return Fail(c, PSP_HIP_ERR_INVALID, "unknown option");
 }
 
 int psp_hip_factor(psp_hip_ctx* c, const double* velocity, int scheme, psp_hip_factor_report* rep)
@@ -761,7 +824,7 @@ int psp_hip_precondition(psp_hip_ctx* c, void* B, int k)
     int rc = CheckSolveArgs(c, B, k, 1);
     if (rc) return rc;
     if ((rc = EnsureWorkspace(c, k, std::max(1, c->wsK == k ? c->wsM : 1)))) return rc;
-    Precondition(c, (c128*)B, k);
+    if ((rc = Precondition(c, (c128*)B, k))) return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PSP_HIP_OK;

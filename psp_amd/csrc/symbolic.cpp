@@ -519,7 +519,7 @@ void Schedule(const Plan& plan, ClassPlan& C)
 
     // Solve schedule.
     {
-        const int R = plan.mvRows, CC = plan.mvCols;
+        const int R = plan.mvRows;
         C.ginv.assign((size_t)2 * C.wstore, -1);
         for (const Node& n : C.nodes)
             for (int k = 0; k < 2; ++k) {
@@ -531,52 +531,25 @@ void Schedule(const Plan& plan, ClassPlan& C)
                 }
             }
         C.solveLevels.assign(C.maxDepth + 1, {});
-        // Adds the tiles of one product: rows [0, nrows) in blocks of R; for
-        // a row block starting at r0 the inner range is [lo(r0), hi(r0)).
-        int groups = 0, slots = 0;
+        // One tile per block of R output rows; for a block starting at r0 the
+        // inner range is [lo(r0), hi(r0)).
         auto addProduct = [&](int t, int nrows, auto lo, auto hi) {
             for (int r0 = 0; r0 < nrows; r0 += R) {
-                const int c0 = lo(r0), c1 = hi(r0);
-                const int nt = c1 > c0 ? (int)CeilDiv(c1 - c0, CC) : 1;
-                for (int s = 0; s < nt; ++s) {
-                    MvTile m;
-                    m.node = t;
-                    m.row0 = r0;
-                    m.nrows = std::min(R, nrows - r0);
-                    m.col0 = c0 + s * CC;
-                    m.col1 = std::min(c1, c0 + (s + 1) * CC);
-                    if (c1 <= c0) m.col0 = m.col1 = c0;
-                    m.slot = s;
-                    m.ntiles = nt;
-                    m.group = groups;
-                    m.part = slots;
-                    m.pad_ = 0;
-                    C.mv.push_back(m);
-                }
-                ++groups;
-                if (nt > 1) slots += nt;
+                MvTile m;
+                m.node = t;
+                m.row0 = r0;
+                m.nrows = std::min(R, nrows - r0);
+                m.col0 = lo(r0);
+                m.col1 = std::max(m.col0, hi(r0));
+                m.pad_ = 0;
+                C.mv.push_back(m);
             }
         };
-        auto beginRange = [&](ClassPlan::Range& r) {
-            r.start = (int)C.mv.size();
-            groups = slots = 0;
-        };
-        auto endRange = [&](ClassPlan::Range& r) {
-            r.count = (int)C.mv.size() - r.start;
-            C.maxGroups = std::max(C.maxGroups, groups);
-            C.maxSlots = std::max(C.maxSlots, slots);
-        };
+        auto beginRange = [&](ClassPlan::Range& r) { r.start = (int)C.mv.size(); };
+        auto endRange = [&](ClassPlan::Range& r) { r.count = (int)C.mv.size() - r.start; };
         for (int d = 0; d <= C.maxDepth; ++d) {
             ClassPlan::SolveLevel& L = C.solveLevels[d];
             const std::vector<int>& lv = C.levels[d];
-            L.f1.start = (int)C.gather.size();
-            for (int t : lv) {
-                const Node& n = C.nodes[t];
-                const int fs = n.size + n.usize;
-                for (int r0 = 0; r0 < fs; r0 += 256)
-                    C.gather.push_back({t, r0, std::min(fs, r0 + 256), 0});
-            }
-            L.f1.count = (int)C.gather.size() - L.f1.start;
             beginRange(L.f2);  // y = Linv W_top, lower triangular
             for (int t : lv)
                 if (C.nodes[t].size > 0) {
@@ -585,10 +558,10 @@ void Schedule(const Plan& plan, ClassPlan& C)
                                [&](int r0) { return std::min(s, r0 + R); });
                 }
             endRange(L.f2);
-            beginRange(L.f3);  // W_bot -= L_BL y
+            beginRange(L.f3);  // W_bot = gathered - L_BL y (also for empty nodes)
             for (int t : lv) {
                 const Node& n = C.nodes[t];
-                if (n.size > 0 && n.usize > 0)
+                if (n.usize > 0)
                     addProduct(t, n.usize, [](int) { return 0; },
                                [&](int) { return n.size; });
             }

@@ -100,22 +100,12 @@ struct InvItem {
     int32_t pad_;
 };
 
-// Solve phase 1 (forward gather): rows [r0, r1) of node `node`'s front
-// work vector W = [Xp_own; 0] + children's update vectors.
-struct GatherItem {
-    int32_t node, r0, r1, pad_;
-};
-
-// One tile of a split-K matrix-vector product of one node: output rows
-// [row0, row0 + nrows) (nrows <= 64), inner indices [col0, col1).  The
-// tiles of one row block form a group; the last tile of a group to finish
-// sums the group's partials in slot order.
+// One row block of a matrix-vector product of one node in a front solve:
+// output rows [row0, row0 + nrows) (nrows <= 64), inner indices [col0, col1).
 struct MvTile {
     int32_t node;
     int32_t row0, nrows;
     int32_t col0, col1;
-    int32_t slot, ntiles, group;
-    int32_t part;  // first partial slot of the group
     int32_t pad_;
 };
 
@@ -173,20 +163,19 @@ struct ClassPlan {
     std::vector<InvItem> inv;
     std::vector<GemmLaunch> invGemm;      // alternating T = C*Ainv, X = -Binv*T
 
-    // Solve schedule.  Per depth: forward gather (F1), forward Linv (F2),
-    // forward update (F3), backward update (B1), backward Linv^T (B2).
+    // Solve schedule.  Per depth: forward Linv (F2, reading the extend-added
+    // right-hand side), forward update (F3), backward update (B1), backward
+    // Linv^T (B2).
     // ginv[2*r + c]: row of child c's work vector feeding front row r (or -1).
     std::vector<int32_t> ginv;
-    std::vector<GatherItem> gather;
     std::vector<MvTile> mv;
     struct Range {
         int start = 0, count = 0;
     };
     struct SolveLevel {
-        Range f1, f2, f3, b1, b2;
+        Range f2, f3, b1, b2;
     };
     std::vector<SolveLevel> solveLevels;  // index = depth
-    int maxGroups = 1, maxSlots = 1;
     std::vector<TransItem> trans;
 
     // Algorithmic work per panel.
@@ -201,7 +190,6 @@ struct Plan {
     int blockNb = 32;      // LDL^T block width
     int tile = 64;         // GEMM tile
     int mvRows = 64;       // output rows per solve tile
-    int mvCols = 256;      // inner indices per solve tile
 };
 
 // Builds everything; returns PSP_HIP_OK or an error code with `msg` set.

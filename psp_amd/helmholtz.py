@@ -185,6 +185,13 @@ class DistHelmholtz:
             torch.cuda.current_stream().synchronize()
         _lib.lib().psp_hip_set_stream(self._ctx(), C.c_void_p(s) if s else None)
 
+    def set_sweep_graph(self, enabled: bool):
+        """Replay the sweep from a captured hipGraph (default) or launch it
+        kernel by kernel."""
+        st = _lib.lib().psp_hip_set_option(self._ctx(), _lib.OPT_SWEEP_GRAPH, int(enabled))
+        if st != _lib.OK:
+            _raise(st, self._h, "set_option")
+
     # ---- Initialize ------------------------------------------------------
     def Initialize(self, velocity, panelScheme=CLIQUE_LDL_SELINV_1D):
         """Factor every panel (initialize_finalize.hpp:12-377)."""

@@ -147,3 +147,28 @@ def test_fp32_fronts_reach_tolerance(torch_cuda):
     assert r32.final_max_rel < 1e-8
     # same operator, both converged to 1e-8: solutions agree to ~cond * 1e-8
     assert _rel(x32, x64) < 1e-5
+
+
+def test_sweep_graph_replays_match_eager_bitwise(torch_cuda):
+    """The captured sweep is replayed several times on one buffer, with the
+    program's own eager work and synchronizations between replays; every
+    replay must equal the eager sweep bit for bit (all reductions in the
+    sweep have a fixed order)."""
+    torch = torch_cuda
+    disc, vel = _problem(16, 14, 26)
+    h = _helm(disc, vel)
+    rng = np.random.default_rng(21)
+    X = torch.from_numpy(rand_block(rng, disc.nx * disc.ny * disc.nz, 3)).cuda()
+    h.set_sweep_graph(False)
+    eager = X.clone()
+    h.Precondition(eager)
+    torch.cuda.synchronize()
+    h.set_sweep_graph(True)
+    y = torch.empty_like(X)
+    for rep in range(4):
+        y.copy_(X * 1.0)
+        junk = torch.randn(1000, device="cuda").sum()  # eager work between replays
+        h.Precondition(y)
+        torch.cuda.synchronize()
+        assert torch.equal(y, eager), rep
+        del junk
