@@ -29,6 +29,68 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic():
+    """HBM bytes per k_mv launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc)
+    recorded in the committed profile summary, or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_mv_front_solves_bumps100_8rhs.md")
+    try:
+        m = re.search(r"\*\*([0-9.]+) MB per launch\*\* of HBM traffic", open(path).read())
+        return float(m.group(1)) * 1e6 if m else None
+    except OSError:
+        return None
+
+
+def cpu_baseline(wl, out, args):
+    """FP64 CPU restatement of PSP's method (the oracle, OpenMP over the host
+    cores) timed on a bounded sample of the same workload, extrapolated to
+    one Solve of the whole right-hand-side block:
+      factor time = sum over panel classes of (#panels x one panel factored),
+      solve time  = iterations x (sweep + 2 operator applications)
+                    + iterations / m x one GMRES(m) Arnoldi cycle,
+      sweep       = sum over the 2P-1 panel solves of the class's panel-solve
+                    time, each measured once on its factored sample panel.
+    The iteration count is the GPU FP64 path's (same method)."""
+    from oracle.oracle import Oracle, num_threads
+    d, vel, B = wl.disc, wl.velocity, wl.B
+    k = B.shape[1]
+    o = Oracle(d.omega, d.nx, d.ny, d.nz, d.wx, d.wy, d.wz, sigma=d.sigmax)
+    cores = num_threads()
+    np_ = o.num_panels
+    info = [o.panel_info(p) for p in range(np_)]
+    rep_panel = {}  # one sample panel per class (a full inner panel for class 0)
+    for p in range(np_ - 1, -1, -1):
+        rep_panel[info[p]["cls"]] = p
+    t_fac = {}
+    for c, p in rep_panel.items():
+        t_fac[c] = o.factor_sample(vel, p, 1)
+        log(f"[cpu] factored panel {p} (class {c}) in {t_fac[c]:.2f}s")
+    o.fill_global(vel)
+    l2n = o.local_to_natural()
+    Bnd = np.asfortranarray(B[l2n])
+    t_ps = {}
+    for c, p in rep_panel.items():
+        t0 = time.perf_counter()
+        o.solve_panel(Bnd, p)
+        t_ps[c] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.multiply(Bnd)
+    t_mv = time.perf_counter() - t0
+    t_arn = o.time_arnoldi(k, args.m)
+    n_fac = sum(t_fac[info[p]["cls"]] for p in range(np_))
+    t_sweep = sum(t_ps[info[p]["cls"]] * (2 if p < np_ - 1 else 1) for p in range(np_))
+    its = out.get("fp64", out["fp32"])["iterations"]
+    t_solve = its * (t_sweep + 2 * t_mv) + (its / args.m) * t_arn + t_sweep
+    log(f"[cpu] sweep {t_sweep:.2f}s, multiply {t_mv:.3f}s, arnoldi cycle {t_arn:.2f}s -> "
+        f"solve {t_solve:.1f}s, factor {n_fac:.1f}s (est.)")
+    return {"value": k / t_solve, "unit": "RHS/s", "cores": cores, "kind": "port",
+            "factor_s_est": n_fac, "solve_s_est": t_solve,
+            "sample": (f"oracle (FP64 CPU restatement, not the PSP binary): one panel of each of "
+                       f"{len(rep_panel)} panel classes factored and solved once with the "
+                       f"{k}-RHS block, one operator application, one GMRES({args.m}) Arnoldi "
+                       f"cycle; extrapolated to {its} iterations and {np_} panels")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -40,6 +102,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-front leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="copy a 1 GiB device buffer once (a known-size kernel for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -63,6 +128,12 @@ def main():
     Bsrc = torch.from_numpy(B).to(dev)
 
     out = {}
+    if args.calibrate:
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     def run_path(precision, steps, warmup):
         h = psp_amd.DistHelmholtz(disc, precision=precision)
@@ -105,6 +176,23 @@ def main():
         out["fp32_vs_fp64_rel_err"] = err
         log(f"[bench] fp32 vs fp64 solution rel err {err:.3e}")
 
+    # ---- roofline of the dominant kernels: the front-solve products -----
+    # One more sweep of the same block, launched kernel by kernel with HIP
+    # events around every front-solve product launch (the timed solves replay
+    # the sweep from a hipGraph, whose kernels cannot be bracketed singly).
+    launches, ksec, kbytes, kflops = h32.ProfileSweep(Bsrc.clone())
+    achieved = kbytes / ksec / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                "frac": achieved / 8000.0, "traffic": measured_traffic(),
+                "kernel": "k_mv (front-solve products, c64 fronts)",
+                "launches_per_sweep": launches, "avg_launch_us": 1e6 * ksec / launches,
+                "algorithmic_bytes_per_launch": kbytes / launches,
+                "achieved_tflops": kflops / ksec / 1e12}
+    log(f"[bench] roofline: {launches} k_mv launches/sweep, avg {1e6 * ksec / launches:.1f} us, "
+        f"{kbytes / launches / 1e6:.2f} MB/launch -> {achieved:.0f} GB/s")
+
+    cpu = None if args.no_cpu else cpu_baseline(wl, out, args)
+
     line = {
         "metric": "RHS solves/s to 1e-5 rel. residual (FP32 fronts + FP64 GMRES)",
         "value": value,
@@ -120,6 +208,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": [disc.nx, disc.ny, disc.nz], "rhs": k,
                    "gmres_m": args.m, "rel_tol": args.tol, "omega": disc.omega},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
         "paths": out,
     }
     print(json.dumps(line), flush=True)

@@ -101,6 +101,14 @@ typedef struct {
     int num_precond, num_multiply;
 } psp_hip_solve_report;
 
+/* Front-solve product kernels of one sweep, timed launch by launch. */
+typedef struct {
+    int launches;        /* kernel launches timed                          */
+    double seconds;      /* summed HIP-event durations of those launches   */
+    double bytes;        /* algorithmic bytes: factor entries + vectors    */
+    double flops;        /* algorithmic real flops (8 per complex MAC)     */
+} psp_hip_kernel_stats;
+
 typedef struct psp_hip_ctx psp_hip_ctx;
 
 /* DistHelmholtz ctor (dist_helmholtz/main.hpp:12-436): panel split, nested
@@ -145,6 +153,12 @@ int psp_hip_precondition(psp_hip_ctx* ctx, void* B, int k);
 /* DistHelmholtz::SolvePanel (solve.hpp:736-882) in place on a device block
  * in grid layout (only panel `panel`'s planes are read and written). */
 int psp_hip_solve_panel(psp_hip_ctx* ctx, void* B, int k, int panel);
+
+/* One Precondition on a device block, launched kernel by kernel with HIP
+ * events around every front-solve product launch (no reference
+ * counterpart; measurement only). */
+int psp_hip_profile_sweep(psp_hip_ctx* ctx, void* B, int k,
+                          psp_hip_kernel_stats* stats);
 
 /* GMRES with a fixed number of inner steps, for parity at a fixed
  * iteration count; relHist (host, steps doubles) gets max_r rel. residual. */

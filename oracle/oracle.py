@@ -74,6 +74,9 @@ def lib():
             "ora_solve": (I, [P, P, I, I, D, I, C.POINTER(Report)]),
             "ora_solve_fixed": (I, [P, P, I, I, I, P]),
             "ora_partial_ldl": (I, [P, I, I]),
+            "ora_fill_global": (I, [P, P]),
+            "ora_num_threads": (I, []),
+            "ora_time_arnoldi": (D, [P, I, I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -173,6 +176,14 @@ class Oracle:
             raise OracleError(st, "factor_sample")
         return t.value
 
+    def fill_global(self, velocity):
+        v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
+        self._vel = v
+        lib().ora_fill_global(self._h, _ptr(v))
+
+    def time_arnoldi(self, k, m):
+        return lib().ora_time_arnoldi(self._h, k, m)
+
     def global_csr(self):
         nnz = lib().ora_global_nnz(self._h)
         rp = np.empty(self.N + 1, np.int32)
@@ -237,6 +248,10 @@ class Oracle:
         if st != 0:
             raise OracleError(st, "solve_fixed")
         return B, hist
+
+
+def num_threads():
+    return lib().ora_num_threads()
 
 
 def partial_ldl(F, s):

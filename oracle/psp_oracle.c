@@ -605,6 +605,7 @@ void ora_destroy(ora_ctx* X)
 }
 
 int ora_num_panels(const ora_ctx* X) { return X->numPanels; }
+int ora_num_threads(void) { return omp_get_max_threads(); }
 long ora_global_size(const ora_ctx* X) { return X->N; }
 
 int ora_panel_info(const ora_ctx* X, int p, int* vPanel, int* depth,
@@ -890,7 +891,7 @@ int ora_factor_sample(ora_ctx* X, const double* vel, int first, int count,
 {
     if (first < 0 || first + count > X->numPanels) return ORA_ERR_INVALID;
     const double t0 = omp_get_wtime();
-#pragma omp parallel for schedule(dynamic, 1)
+    /* one panel at a time so the dense front kernels use every thread */
     for (int p = first; p < first + count; ++p) factor_panel(X, p, vel);
     if (seconds) *seconds = omp_get_wtime() - t0;
     return ORA_OK;
@@ -910,6 +911,13 @@ int ora_factor(ora_ctx* X, const double* vel)
 }
 
 double ora_factor_seconds(const ora_ctx* X) { return X->tfactor; }
+
+int ora_fill_global(ora_ctx* X, const double* vel)
+{
+    fill_global(X, vel);
+    X->factored = 1;
+    return ORA_OK;
+}
 
 /* ------------------------------------------------------------------ */
 /* Multifrontal triangular solves (cliq::Solve, call solve.hpp:822):   */
@@ -1339,6 +1347,47 @@ done:
     free(alpha);
     free(y);
     return status;
+}
+
+double ora_time_arnoldi(const ora_ctx* X, int k, int m)
+{
+    const long N = X->N;
+    zc* V = (zc*)malloc(sizeof(zc) * (size_t)(N * k * m));
+    zc* w = (zc*)malloc(sizeof(zc) * (size_t)(N * k));
+    zc* x = (zc*)malloc(sizeof(zc) * (size_t)(N * k));
+    zc* alpha = (zc*)malloc(sizeof(zc) * (size_t)k);
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < N * k * m; ++q) V[q] = (double)((q * 2654435761u) % 1000) * 1e-3;
+    for (long q = 0; q < N * k; ++q) w[q] = 1.0;
+    const double t0 = omp_get_wtime();
+    for (int j = 0; j < m; ++j) {
+        for (int i = 0; i <= j; ++i) {
+#pragma omp parallel for schedule(static) if (k > 1)
+            for (int r = 0; r < k; ++r) {
+                const zc* vi = V + ((long)i * k + r) * N;
+                zc* wr = w + r * N;
+                const zc a = dotc(vi, wr, N) * 1e-6;
+                alpha[r] = a;
+                for (long q = 0; q < N; ++q) wr[q] -= a * vi[q];
+            }
+        }
+        for (int r = 0; r < k; ++r) alpha[r] = nrm2(w + r * N, N);
+#pragma omp parallel for schedule(static) if (k > 1)
+        for (int r = 0; r < k; ++r) {
+            zc* xr = x + r * N;
+            for (long q = 0; q < N; ++q) xr[q] = w[r * N + q];
+            for (int i = 0; i <= j; ++i) {
+                const zc* vi = V + ((long)i * k + r) * N;
+                for (long q = 0; q < N; ++q) xr[q] += 1e-3 * vi[q];
+            }
+        }
+    }
+    const double t = omp_get_wtime() - t0;
+    free(V);
+    free(w);
+    free(x);
+    free(alpha);
+    return t;
 }
 
 /* PullRightHandSides / PushRightHandSides (solve.hpp:58-239). */

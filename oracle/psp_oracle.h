@@ -60,6 +60,7 @@ typedef struct {
 typedef struct ora_ctx ora_ctx;
 
 int  ora_create(const ora_disc* disc, ora_ctx** out);
+int  ora_num_threads(void);  /* OpenMP threads the oracle runs on */
 void ora_destroy(ora_ctx* ctx);
 
 /* Geometry queries (main.hpp:35-57, 728-821). */
@@ -102,6 +103,15 @@ double ora_factor_seconds(const ora_ctx* ctx);
 /* Factor only `count` panels starting at `first` (CPU baseline sampling). */
 int  ora_factor_sample(ora_ctx* ctx, const double* velocity, int first,
                        int count, double* seconds);
+
+/* Build only the global operator (the last block of Initialize), for CPU
+ * baseline sampling. */
+int  ora_fill_global(ora_ctx* ctx, const double* velocity);
+
+/* Seconds for one restart cycle of the GMRES(m) Arnoldi work on k columns:
+ * the modified Gram-Schmidt steps j = 0..m-1 plus the x := x0 + V y updates
+ * (solve.hpp:382-399, 512-521), on random data (CPU baseline sampling). */
+double ora_time_arnoldi(const ora_ctx* ctx, int k, int m);
 
 /* Operators on a global ND-ordered, column-major N x k block (ld = N).
  * B interleaved complex (re, im). */

@@ -24,6 +24,7 @@ EXPORTS = (
     "psp_hip_set_option",
     "psp_hip_factor", "psp_hip_solve", "psp_hip_multiply", "psp_hip_precondition",
     "psp_hip_solve_panel", "psp_hip_solve_fixed", "psp_hip_adjoint_gradient",
+    "psp_hip_profile_sweep",
     "psp_hip_num_panels", "psp_hip_panel_info", "psp_hip_class_num_nodes",
     "psp_hip_class_nodes", "psp_hip_reordered_index", "psp_hip_device_count",
 )
@@ -53,6 +54,11 @@ class FactorReport(C.Structure):
         ("bytes_factor", C.c_double),
         ("num_panels", C.c_int), ("num_fronts", C.c_int),
     ]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_int), ("seconds", C.c_double), ("bytes", C.c_double),
+                ("flops", C.c_double)]
 
 
 class SolveReport(C.Structure):
@@ -93,6 +99,7 @@ def lib():
         "psp_hip_solve_panel": (I, [P, P, I, I]),
         "psp_hip_solve_fixed": (I, [P, P, I, I, I, I, P]),
         "psp_hip_adjoint_gradient": (I, [P, P, P, I, P, I]),
+        "psp_hip_profile_sweep": (I, [P, P, I, C.POINTER(KernelStats)]),
         "psp_hip_num_panels": (I, [P]),
         "psp_hip_panel_info": (I, [P, I, IP, IP, IP, IP]),
         "psp_hip_class_num_nodes": (I, [P, I]),

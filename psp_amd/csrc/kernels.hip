@@ -941,27 +941,39 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s)
 
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, int k,
-                      hipStream_t s)
+                      hipStream_t s, MvProfile* prof)
 {
     const cx<T>* L = (const cx<T>*)c.L + (int64_t)b * c.fstore;
     const cx<T>* LT = (const cx<T>*)c.LT + (int64_t)b * c.fstore;
     cx<T>* Xp = (cx<T>*)Xpv;
     cx<T>* W = (cx<T>*)Wv;
-    auto mv = [&](auto ph, const ClassPlan::Range& r) {
+    auto mv = [&](auto ph, const ClassPlan::SolveLevel& lv, const ClassPlan::Range& r) {
         if (r.count <= 0) return;
         constexpr int PH = decltype(ph)::value;
+        if (prof) (void)hipEventRecord(prof->a, s);
         hipLaunchKernelGGL((k_mv<T, PH>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv, r.start,
                            c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
+        if (prof) {
+            (void)hipEventRecord(prof->b, s);
+            (void)hipEventSynchronize(prof->b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, prof->a, prof->b);
+            prof->launches += 1;
+            prof->seconds += ms * 1e-3;
+            const double eb = sizeof(cx<T>);
+            prof->bytes += eb * (lv.entries[PH] + lv.vecRows[PH] * k);
+            prof->flops += 8.0 * lv.entries[PH] * k;
+        }
     };
     for (int d = plan.maxDepth; d >= 0; --d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
-        mv(std::integral_constant<int, PH_F2>(), lv.f2);
-        mv(std::integral_constant<int, PH_F3>(), lv.f3);
+        mv(std::integral_constant<int, PH_F2>(), lv, lv.f2);
+        mv(std::integral_constant<int, PH_F3>(), lv, lv.f3);
     }
     for (int d = 0; d <= plan.maxDepth; ++d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
-        mv(std::integral_constant<int, PH_B1>(), lv.b1);
-        mv(std::integral_constant<int, PH_B2>(), lv.b2);
+        mv(std::integral_constant<int, PH_B1>(), lv, lv.b1);
+        mv(std::integral_constant<int, PH_B2>(), lv, lv.b2);
     }
 }
 
@@ -1078,7 +1090,7 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
     template void LaunchGemm<T>(const ClassDev&, int, int, hipStream_t);                          \
     template void LaunchInvDiag<T>(const ClassDev&, int, hipStream_t);                            \
     template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, int,  \
-                                      hipStream_t);                                               \
+                                      hipStream_t, MvProfile*);                                   \
     template void LaunchTranspose<T>(const ClassDev&, hipStream_t);                               \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
                                   int, hipStream_t);                                              \

@@ -84,11 +84,19 @@ void LaunchGemm(const ClassDev& c, int tileStart, int tileCount, hipStream_t s);
 template <typename T>
 void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s);
 
+// Optional timing of the front-solve product launches: HIP events around
+// every launch (synchronizing after each), plus their algorithmic work.
+struct MvProfile {
+    hipEvent_t a = nullptr, b = nullptr;
+    int launches = 0;
+    double seconds = 0, bytes = 0, flops = 0;
+};
+
 // Solve of one panel (batch entry b of class c) on the padded ND vector Xp
 // (n x k, k interleaved) with node work vectors W (wstore x k).
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xp, void* W,
-                      int k, hipStream_t s);
+                      int k, hipStream_t s, MvProfile* prof = nullptr);
 template <typename T>
 void LaunchTranspose(const ClassDev& c, hipStream_t s);
 // Natural-order <-> padded panel vector.  mode 0: B = Xp, 1: B += Xp.

@@ -391,19 +391,19 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv)
 // Sweep: Precondition (solve.hpp:681-734) on a natural-order device block.
 // ---------------------------------------------------------------------------
 template <typename T>
-void PanelForward(psp_hip_ctx* c, c128* B, int k, int p)
+void PanelForward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = nullptr)
 {
     const Geometry& g = c->plan.geo;
     const int cl = g.cls(p);
     const ClassDev& D = c->dev[cl];
     hipStream_t s = c->stream;
     LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s, prof);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s);
 }
 
 template <typename T>
-void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p)
+void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = nullptr)
 {
     // ExtractPanel + MultiplySuperdiagonal + SolvePanel + UpdatePanel
     // (solve.hpp:725-733): B_i still holds Z, so the solve of the coupling
@@ -416,21 +416,21 @@ void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p)
     const int v2 = g.panelV(p) + g.depth(p) - 1;
     const int vlLast = g.padding(p) + g.depth(p) - 1;
     LaunchSupdiagSet<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s, prof);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 1, s);
 }
 
 template <typename T>
-void Sweep(psp_hip_ctx* c, c128* B, int k)
+void Sweep(psp_hip_ctx* c, c128* B, int k, MvProfile* prof = nullptr)
 {
     const Geometry& g = c->plan.geo;
     const StencilArgs st = Stencil(c);
     for (int i = 0; i < g.numPanels - 1; ++i) {
-        PanelForward<T>(c, B, k, i);
+        PanelForward<T>(c, B, k, i, prof);
         LaunchSubdiag(st, c->vel, B, k, g.panelV(i + 1), c->stream);
     }
-    PanelForward<T>(c, B, k, g.numPanels - 1);
-    for (int i = g.numPanels - 2; i >= 0; --i) PanelBackward<T>(c, B, k, i);
+    PanelForward<T>(c, B, k, g.numPanels - 1, prof);
+    for (int i = g.numPanels - 2; i >= 0; --i) PanelBackward<T>(c, B, k, i, prof);
 }
 
 void SweepEager(psp_hip_ctx* c, c128* B, int k)
@@ -827,6 +827,31 @@ int psp_hip_precondition(psp_hip_ctx* c, void* B, int k)
     if ((rc = Precondition(c, (c128*)B, k))) return rc;
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSP_HIP_OK;
+}
+
+int psp_hip_profile_sweep(psp_hip_ctx* c, void* B, int k, psp_hip_kernel_stats* stats)
+{
+    int rc = CheckSolveArgs(c, B, k, 1);
+    if (rc) return rc;
+    if (!stats) return Fail(c, PSP_HIP_ERR_INVALID, "null stats");
+    if ((rc = EnsureWorkspace(c, k, std::max(1, c->wsK == k ? c->wsM : 1)))) return rc;
+    MvProfile prof;
+    HIPCHK(c, hipEventCreate(&prof.a));
+    HIPCHK(c, hipEventCreate(&prof.b));
+    if (c->desc.precision == PSP_HIP_FRONTS_FP32)
+        Sweep<float>(c, (c128*)B, k, &prof);
+    else
+        Sweep<double>(c, (c128*)B, k, &prof);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipEventDestroy(prof.a);
+    (void)hipEventDestroy(prof.b);
+    HIPCHK(c, e);
+    HIPCHK(c, hipGetLastError());
+    stats->launches = prof.launches;
+    stats->seconds = prof.seconds;
+    stats->bytes = prof.bytes;
+    stats->flops = prof.flops;
     return PSP_HIP_OK;
 }
 

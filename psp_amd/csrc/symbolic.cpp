@@ -547,6 +547,19 @@ void Schedule(const Plan& plan, ClassPlan& C)
         };
         auto beginRange = [&](ClassPlan::Range& r) { r.start = (int)C.mv.size(); };
         auto endRange = [&](ClassPlan::Range& r) { r.count = (int)C.mv.size() - r.start; };
+        auto account = [&](ClassPlan::SolveLevel& L, int ph, const ClassPlan::Range& r) {
+            for (int i = r.start; i < r.start + r.count; ++i) {
+                const MvTile& m = C.mv[i];
+                double e = 0;
+                for (int row = m.row0; row < m.row0 + m.nrows; ++row) {
+                    if (ph == 0) e += std::min(row, m.col1) - m.col0;        // strictly lower
+                    else if (ph == 3) e += m.col1 - std::max(row + 1, m.col0);  // strictly upper
+                    else e += m.col1 - m.col0;
+                }
+                L.entries[ph] += e;
+                L.vecRows[ph] += (m.col1 - m.col0) + m.nrows;
+            }
+        };
         for (int d = 0; d <= C.maxDepth; ++d) {
             ClassPlan::SolveLevel& L = C.solveLevels[d];
             const std::vector<int>& lv = C.levels[d];
@@ -583,6 +596,10 @@ void Schedule(const Plan& plan, ClassPlan& C)
                 }
             }
             endRange(L.b2);
+            account(L, 0, L.f2);
+            account(L, 1, L.f3);
+            account(L, 2, L.b1);
+            account(L, 3, L.b2);
         }
         for (int t = 0; t < nn; ++t) {
             const Node& n = C.nodes[t];

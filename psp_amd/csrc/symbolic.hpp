@@ -174,6 +174,10 @@ struct ClassPlan {
     };
     struct SolveLevel {
         Range f2, f3, b1, b2;
+        // per phase (F2, F3, B1, B2): factor entries read and vector rows
+        // (inner inputs + outputs) moved, for algorithmic bytes and flops
+        double entries[4] = {0, 0, 0, 0};
+        double vecRows[4] = {0, 0, 0, 0};
     };
     std::vector<SolveLevel> solveLevels;  // index = depth
     std::vector<TransItem> trans;

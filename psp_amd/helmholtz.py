@@ -276,6 +276,17 @@ class DistHelmholtz:
         if st != _lib.OK:
             _raise(st, self._h, "SolvePanel")
 
+    def ProfileSweep(self, B):
+        """One sweep launched kernel by kernel with HIP events around every
+        front-solve product launch; returns (launches, seconds, bytes, flops)."""
+        self._bind_stream()
+        st_ = _lib.KernelStats()
+        st = _lib.lib().psp_hip_profile_sweep(self._ctx(), C.c_void_p(B.data_ptr()), B.shape[1],
+                                              C.byref(st_))
+        if st != _lib.OK:
+            _raise(st, self._h, "ProfileSweep")
+        return st_.launches, st_.seconds, st_.bytes, st_.flops
+
     def AdjointGradient(self, u, lam, grad, accumulate=False):
         self._bind_stream()
         st = _lib.lib().psp_hip_adjoint_gradient(self._ctx(), C.c_void_p(u.data_ptr()),
