@@ -472,17 +472,18 @@ __device__ __forceinline__ void MvFinal(const NodeDev& nd, const int32_t* ginv, 
 
 constexpr int kMvThreads = 512;  // 8 waves split each row block's inner indices
 
-// One workgroup per (node, block of 64 output rows); the inner index range
-// streams through LDS in chunks of kMvCols and the 8 waves' partial sums
-// are combined by a tree reduction in LDS, so every output element is
-// summed in a fixed order.
-template <typename T, int PH>
+// One workgroup per (node, block of RT output rows); the inner index range
+// streams through LDS in chunks of kMvCols.  Lane l of a wave owns row
+// l % RT and column slot l / RT (64 / RT columns per wave step); the slots
+// are combined by lane shuffles and the 8 waves' partial sums by a fixed-
+// order tree in LDS, so every output element is summed in a fixed order.
+template <typename T, int PH, int RT>
 __global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int start,
                                                     const NodeDev* nodes, const int32_t* structs,
                                                     const int32_t* ginv, const cx<T>* L,
                                                     const cx<T>* LT, cx<T>* Xp, cx<T>* W, int k)
 {
-    constexpr int KC = 8, R = 64, CC = kMvCols, NW = kMvThreads / 64, U = 16;
+    constexpr int KC = 8, CC = kMvCols, NW = kMvThreads / 64, U = 16, CPW = 64 / RT;
     const MvTile tl = tiles[start + blockIdx.x];
     const NodeDev nd = nodes[tl.node];
     const int s = nd.s, fs = nd.s + nd.u;
@@ -490,11 +491,14 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int star
     const cx<T>* Lt = LT + nd.l;
     const cx<T>* Wt = W + nd.w * k;
     __shared__ cx<T> vs[CC][KC];  // staged vector chunk; reused for the wave reduction
-    static_assert(CC * KC >= (NW / 2) * R * KC, "reduction scratch must fit in vs");
-    cx<T>(*red)[R][KC] = reinterpret_cast<cx<T>(*)[R][KC]>(&vs[0][0]);
+    static_assert(CC * KC >= (NW / 2) * RT * KC, "reduction scratch must fit in vs");
+    cx<T>(*red)[RT][KC] = reinterpret_cast<cx<T>(*)[RT][KC]>(&vs[0][0]);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int row = tl.row0 + lane;
-    const bool rowOk = lane < tl.nrows;
+    const int rl = lane % RT, slot = lane / RT;
+    const int row = tl.row0 + rl;
+    const bool rowOk = rl < tl.nrows;
+    const int first = slot + CPW * wave;  // this lane's first column in a chunk
+    constexpr int STRIDE = CPW * NW;      // and its column stride
     for (int r0 = 0; r0 < k; r0 += KC) {
         const int kc = min(KC, k - r0);
         cx<T> acc[KC];
@@ -518,12 +522,12 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int star
                 vs[c][r] = v;
             }
             __syncthreads();
-            const int nIter = ncol > wave ? (ncol - wave + NW - 1) / NW : 0;
+            const int nIter = ncol > first ? (ncol - first + STRIDE - 1) / STRIDE : 0;
             for (int i0 = 0; i0 < nIter; i0 += U) {
                 cx<T> m[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int col = cb + wave + NW * (i0 + u);
+                    const int col = cb + first + STRIDE * (i0 + u);
                     cx<T> v = czero<T>();
                     if (rowOk && i0 + u < nIter) {
                         if (PH == PH_F2)
@@ -542,26 +546,35 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int star
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     if (i0 + u < nIter) {
-                        const int c = wave + NW * (i0 + u);
+                        const int c = first + STRIDE * (i0 + u);
 #pragma unroll
                         for (int r = 0; r < KC; ++r) acc[r] = cfma(m[u], vs[c][r], acc[r]);
                     }
                 }
             }
         }
-        for (int h = NW / 2; h >= 1; h >>= 1) {
-            __syncthreads();
-            if (wave >= h && wave < 2 * h) {
+        // combine the column slots of a wave (lanes l, l + RT, ...)
 #pragma unroll
-                for (int r = 0; r < KC; ++r) red[wave - h][lane][r] = acc[r];
-            }
-            __syncthreads();
-            if (wave < h) {
+        for (int o = RT; o < 64; o <<= 1) {
 #pragma unroll
-                for (int r = 0; r < KC; ++r) acc[r] = acc[r] + red[wave][lane][r];
+            for (int r = 0; r < KC; ++r) {
+                acc[r].r += __shfl_xor(acc[r].r, o, 64);
+                acc[r].i += __shfl_xor(acc[r].i, o, 64);
             }
         }
-        if (wave == 0 && rowOk) {
+        for (int h = NW / 2; h >= 1; h >>= 1) {
+            __syncthreads();
+            if (wave >= h && wave < 2 * h && slot == 0) {
+#pragma unroll
+                for (int r = 0; r < KC; ++r) red[wave - h][rl][r] = acc[r];
+            }
+            __syncthreads();
+            if (wave < h && slot == 0) {
+#pragma unroll
+                for (int r = 0; r < KC; ++r) acc[r] = acc[r] + red[wave][rl][r];
+            }
+        }
+        if (wave == 0 && slot == 0 && rowOk) {
 #pragma unroll
             for (int r = 0; r < KC; ++r)
                 if (r < kc) MvFinal<T, PH>(nd, ginv, Lf, Xp, W, row, r0 + r, k, acc[r]);
@@ -951,8 +964,15 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
         if (r.count <= 0) return;
         constexpr int PH = decltype(ph)::value;
         if (prof) (void)hipEventRecord(prof->a, s);
-        hipLaunchKernelGGL((k_mv<T, PH>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv, r.start,
-                           c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
+        if (r.rows == 64)
+            hipLaunchKernelGGL((k_mv<T, PH, 64>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
+                               r.start, c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
+        else if (r.rows == 32)
+            hipLaunchKernelGGL((k_mv<T, PH, 32>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
+                               r.start, c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
+        else
+            hipLaunchKernelGGL((k_mv<T, PH, 16>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
+                               r.start, c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
         if (prof) {
             (void)hipEventRecord(prof->b, s);
             (void)hipEventSynchronize(prof->b);

@@ -532,21 +532,53 @@ void Schedule(const Plan& plan, ClassPlan& C)
             }
         C.solveLevels.assign(C.maxDepth + 1, {});
         // One tile per block of R output rows; for a block starting at r0 the
-        // inner range is [lo(r0), hi(r0)).
-        auto addProduct = [&](int t, int nrows, auto lo, auto hi) {
-            for (int r0 = 0; r0 < nrows; r0 += R) {
-                MvTile m;
-                m.node = t;
-                m.row0 = r0;
-                m.nrows = std::min(R, nrows - r0);
-                m.col0 = lo(r0);
-                m.col1 = std::max(m.col0, hi(r0));
-                m.pad_ = 0;
-                C.mv.push_back(m);
-            }
+        // inner range is [lo(r0), hi(r0)).  Per (level, phase) R is the
+        // largest of 64, 32, 16 that still gives kMinTiles workgroups, so
+        // the few big fronts near the root spread over more of the chip.
+        constexpr int kMinTiles = 256;
+        (void)R;
+        auto nrowsOf = [&](int ph, const Node& n) {
+            return (ph == 1) ? n.usize : n.size;
         };
-        auto beginRange = [&](ClassPlan::Range& r) { r.start = (int)C.mv.size(); };
-        auto endRange = [&](ClassPlan::Range& r) { r.count = (int)C.mv.size() - r.start; };
+        auto wanted = [&](int ph, const Node& n) {
+            if (ph == 1) return n.usize > 0;  // F3 also for empty nodes
+            return n.size > 0;
+        };
+        auto loOf = [&](int ph, const Node&, int r0) { return ph == 3 ? r0 : 0; };
+        auto hiOf = [&](int ph, const Node& n, int r0, int rows) {
+            if (ph == 0) return std::min(n.size, r0 + rows);
+            if (ph == 1) return n.size;
+            if (ph == 2) return n.usize;
+            return n.size;
+        };
+        auto build = [&](int ph, const std::vector<int>& lv, ClassPlan::Range& rg) {
+            int rows = 64;
+            for (int cand : {64, 32, 16}) {
+                int64_t tiles = 0;
+                for (int t : lv)
+                    if (wanted(ph, C.nodes[t])) tiles += CeilDiv(nrowsOf(ph, C.nodes[t]), cand);
+                rows = cand;
+                if (tiles >= kMinTiles) break;
+            }
+            rg.rows = rows;
+            rg.start = (int)C.mv.size();
+            for (int t : lv) {
+                const Node& n = C.nodes[t];
+                if (!wanted(ph, n)) continue;
+                const int nr = nrowsOf(ph, n);
+                for (int r0 = 0; r0 < nr; r0 += rows) {
+                    MvTile m;
+                    m.node = t;
+                    m.row0 = r0;
+                    m.nrows = std::min(rows, nr - r0);
+                    m.col0 = loOf(ph, n, r0);
+                    m.col1 = std::max(m.col0, hiOf(ph, n, r0, rows));
+                    m.pad_ = 0;
+                    C.mv.push_back(m);
+                }
+            }
+            rg.count = (int)C.mv.size() - rg.start;
+        };
         auto account = [&](ClassPlan::SolveLevel& L, int ph, const ClassPlan::Range& r) {
             for (int i = r.start; i < r.start + r.count; ++i) {
                 const MvTile& m = C.mv[i];
@@ -563,39 +595,10 @@ void Schedule(const Plan& plan, ClassPlan& C)
         for (int d = 0; d <= C.maxDepth; ++d) {
             ClassPlan::SolveLevel& L = C.solveLevels[d];
             const std::vector<int>& lv = C.levels[d];
-            beginRange(L.f2);  // y = Linv W_top, lower triangular
-            for (int t : lv)
-                if (C.nodes[t].size > 0) {
-                    const int s = C.nodes[t].size;
-                    addProduct(t, s, [](int) { return 0; },
-                               [&](int r0) { return std::min(s, r0 + R); });
-                }
-            endRange(L.f2);
-            beginRange(L.f3);  // W_bot = gathered - L_BL y (also for empty nodes)
-            for (int t : lv) {
-                const Node& n = C.nodes[t];
-                if (n.usize > 0)
-                    addProduct(t, n.usize, [](int) { return 0; },
-                               [&](int) { return n.size; });
-            }
-            endRange(L.f3);
-            beginRange(L.b1);  // W_top = D^-1 y - L_BL^T x_struct
-            for (int t : lv) {
-                const Node& n = C.nodes[t];
-                if (n.size > 0)
-                    addProduct(t, n.size, [](int) { return 0; },
-                               [&](int) { return n.usize; });
-            }
-            endRange(L.b1);
-            beginRange(L.b2);  // Xp_own = Linv^T W_top, upper triangular
-            for (int t : lv) {
-                const Node& n = C.nodes[t];
-                if (n.size > 0) {
-                    const int s = n.size;
-                    addProduct(t, s, [](int r0) { return r0; }, [&](int) { return s; });
-                }
-            }
-            endRange(L.b2);
+            build(0, lv, L.f2);  // y = Linv W_top, lower triangular
+            build(1, lv, L.f3);  // W_bot = gathered - L_BL y (also for empty nodes)
+            build(2, lv, L.b1);  // W_top = D^-1 y - L_BL^T x_struct
+            build(3, lv, L.b2);  // Xp_own = Linv^T W_top, upper triangular
             account(L, 0, L.f2);
             account(L, 1, L.f3);
             account(L, 2, L.b1);

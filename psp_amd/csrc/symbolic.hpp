@@ -171,6 +171,7 @@ struct ClassPlan {
     std::vector<MvTile> mv;
     struct Range {
         int start = 0, count = 0;
+        int rows = 64;  // output rows per tile (64, 32 or 16)
     };
     struct SolveLevel {
         Range f2, f3, b1, b2;
