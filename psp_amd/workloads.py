@@ -157,3 +157,48 @@ def config3(nx=200, ny=200, nz=100, k=64):
     centers = [(x, y, z0) for y in ys for x in xs][:k]
     B = gaussian_sources(nx, ny, nz, centers)
     return Workload(f"thrust-belt {nx}x{ny}x{nz} 8ppw {k}rhs", disc, vel, B)
+
+
+@dataclass
+class InversionWorkload:
+    """Misfit workload for the adjoint-state gradient: squared slowness m of
+    the current model, the true model's m_true, sources (N, k) and receiver
+    indices (natural order) on a surface plane."""
+    name: str
+    disc: Discretization
+    m: np.ndarray
+    m_true: np.ndarray
+    B: np.ndarray
+    receivers: np.ndarray
+
+
+def surface_survey(nx, ny, nz, k, pml=5, rec_stride=2):
+    """k sources on a near-square lattice of the plane z = (pml + 2) h and
+    receivers on every rec_stride-th point of the plane z = pml + 3, both
+    kept off the lateral PML where the grid is wide enough."""
+    h = 1.0 / (max(nx, ny, nz) + 1)
+    sx = int(np.ceil(np.sqrt(k)))
+    sy = int(np.ceil(k / sx))
+    xs = np.linspace(0.2, 0.8, sx) * nx * h
+    ys = np.linspace(0.2, 0.8, sy) * ny * h
+    centers = [(x, y, (pml + 2) * h) for y in ys for x in xs][:k]
+    B = gaussian_sources(nx, ny, nz, centers)
+    zr = pml + 3
+    lx, ly = min(pml + 1, nx // 3), min(pml + 1, ny // 3)
+    rx = np.arange(lx, nx - lx, rec_stride)
+    ry = np.arange(ly, ny - ly, rec_stride)
+    yy, xx = np.meshgrid(ry, rx, indexing="ij")
+    receivers = (xx + yy * nx + zr * nx * ny).reshape(-1).astype(np.int64)
+    return B, receivers
+
+
+def config5(n=80, k=16):
+    """80^3 adjoint-gradient case: true model = Gaussian bumps in [1.5, 4.5],
+    current model = the constant background 3.0, 10 PPW at c = 1.5,
+    16 surface sources, receivers on every second point of a surface plane."""
+    vel_true = bump_velocity(n, n, n, cfg=5)
+    disc = discretization(n, n, n, 1.5, 10.0)
+    B, rec = surface_survey(n, n, n, k)
+    m = np.full(n ** 3, 1.0 / 9.0)
+    return InversionWorkload(f"adjoint bumps {n}^3 10ppw {k}src", disc, m, 1.0 / vel_true ** 2,
+                             B, rec)
