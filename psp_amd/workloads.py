@@ -192,13 +192,26 @@ def surface_survey(nx, ny, nz, k, pml=5, rec_stride=2):
     return B, receivers
 
 
-def config5(n=80, k=16):
-    """80^3 adjoint-gradient case: true model = Gaussian bumps in [1.5, 4.5],
-    current model = the constant background 3.0, 10 PPW at c = 1.5,
-    16 surface sources, receivers on every second point of a surface plane."""
-    vel_true = bump_velocity(n, n, n, cfg=5)
-    disc = discretization(n, n, n, 1.5, 10.0)
-    B, rec = surface_survey(n, n, n, k)
-    m = np.full(n ** 3, 1.0 / 9.0)
-    return InversionWorkload(f"adjoint bumps {n}^3 10ppw {k}src", disc, m, 1.0 / vel_true ** 2,
-                             B, rec)
+def config5(n=80, k=16, pml=5):
+    """80^3 adjoint-gradient case.  Current model: Gaussian bumps in
+    [1.5, 4.5] (the 100^3 recipe, its own seed); true model: the same with a
+    +10% velocity anomaly, a Gaussian of radius 10 cells at the centre.
+    10 PPW at the current model's c_min.  16 sources on a 4x4 lattice of the
+    first non-PML plane below the top (z = pml; the top face is z = 0, PSP's
+    v = nz - 1), receivers at every (x, y) of the last non-PML plane above
+    the bottom (z = nz - 1 - pml)."""
+    vel = bump_velocity(n, n, n, cfg=5)
+    xx, yy, zz, h = grid_coords(n, n, n)
+    c0 = (n - 1) / 2 * h
+    r = 10 * h
+    anomaly = np.exp(-((xx - c0) ** 2 + (yy - c0) ** 2 + (zz - c0) ** 2) / (2 * r * r)).reshape(-1)
+    vel_true = vel * (1 + 0.1 * anomaly)
+    disc = discretization(n, n, n, float(vel.min()), 10.0)
+    side = int(np.ceil(np.sqrt(k)))
+    xs = np.linspace(0.2, 0.8, side) * n * h
+    centers = [(x, y, pml * h) for y in xs for x in xs][:k]
+    B = gaussian_sources(n, n, n, centers)
+    zr = n - 1 - pml
+    receivers = np.arange(zr * n * n, (zr + 1) * n * n, dtype=np.int64)
+    return InversionWorkload(f"adjoint bumps {n}^3 10ppw {k}src", disc, 1.0 / vel ** 2,
+                             1.0 / vel_true ** 2, B, receivers)

@@ -49,3 +49,20 @@ def gradient(o, m, F, rec, data, conj_residual=True):
     dd = diag_derivative(o, m)
     g = -(dd * (Lam * U).sum(axis=1)).real
     return 0.5 * np.vdot(r, r).real, g
+
+
+def gradient_oracle(o, m, F, rec, data, tol=1e-12, restart=20):
+    """(loss, dL/dm) with the oracle's FP64 sweep-preconditioned GMRES for
+    the forward and adjoint solves, for grids too large for sparse LU."""
+    vel = 1.0 / np.sqrt(m)
+    dd = diag_derivative(o, m)
+    o.factor(vel)
+    U, st, _ = o.solve(F, restart, tol)
+    assert st == 0
+    r = U[rec] - data
+    rhs = np.zeros_like(U)
+    rhs[rec] = np.conj(r)
+    Lam, st, _ = o.solve(rhs, restart, tol)
+    assert st == 0
+    g = -(dd * (Lam * U).sum(axis=1)).real
+    return 0.5 * np.vdot(r, r).real, g

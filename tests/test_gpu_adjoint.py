@@ -5,9 +5,12 @@
 - the whole gradient (forward solve, adjoint solve reusing the factors,
   kernel) against the FP64 CPU reference of tests/adjref.py (exact sparse
   LU solves): 1e-7 relative with FP64 fronts and GMRES to 1e-11;
-- the 80^3, 16-source surface survey: directional derivatives against
-  central finite differences of the GPU misfit, 1e-4 relative, with FP64
-  fronts; FP32 fronts (FP64 GMRES) agree with FP64 fronts to 1e-4.
+- the same survey at 40^3 against the FP64 CPU gradient built on the
+  oracle's GMRES solves: 1e-8 relative;
+- the 80^3, 16-source survey: directional derivatives along 8 seeded unit
+  directions against central finite differences of the GPU misfit, 1e-4
+  relative, with FP64 fronts; FP32 fronts (FP64 GMRES) agree with FP64
+  fronts to 1e-4.
 """
 import numpy as np
 import pytest
@@ -71,26 +74,52 @@ def test_gradient_matches_cpu_reference(torch_cuda, precision, tol):
     assert _rel(g.cpu().numpy(), g_ref) < tol
 
 
+def _survey_data(torch, wl, tol=1e-11):
+    """Receiver data of the true model, solved on the GPU in FP64."""
+    import psp_amd
+    ht = psp_amd.DistHelmholtz(wl.disc)
+    ht.Initialize(1.0 / np.sqrt(wl.m_true))
+    Ut = torch.from_numpy(wl.B).cuda()
+    ht.Solve(Ut, 20, tol, 1000)
+    rec = torch.from_numpy(wl.receivers).cuda()
+    return Ut[rec].clone()
+
+
+def test_gradient_40cube_matches_oracle(torch_cuda):
+    """The 80^3 survey recipe at 40^3, 4 sources: GPU gradient against the
+    FP64 CPU gradient of the oracle's own GMRES solves (both to 1e-12)."""
+    torch = torch_cuda
+    from psp_amd import workloads
+    from psp_amd.adjoint import misfit_and_gradient
+    from oracle.oracle import Oracle
+    wl = workloads.config5(n=40, k=4)
+    d = wl.disc
+    data = _survey_data(torch, wl, 1e-12)
+    o = Oracle(d.omega, d.nx, d.ny, d.nz, d.wx, d.wy, d.wz, sigma=d.sigmax)
+    loss_ref, g_ref = adjref.gradient_oracle(o, wl.m, wl.B, wl.receivers, data.cpu().numpy())
+    opts = {"m": 20, "tol": 1e-12, "maxIts": 800, "precision": "fp64"}
+    loss, g = misfit_and_gradient(torch.from_numpy(wl.m).cuda(), torch.from_numpy(wl.B).cuda(),
+                                  torch.from_numpy(wl.receivers).cuda(), data, d, opts)
+    err = _rel(g.cpu().numpy(), g_ref)
+    print(f"[adjoint 40^3] loss {loss.item():.9e} oracle {loss_ref:.9e} grad rel err {err:.2e}",
+          flush=True)
+    assert abs(loss.item() - loss_ref) <= 1e-8 * loss_ref
+    assert err < 1e-8
+
+
 def test_gradient_80cube_central_fd(torch_cuda):
+    """80^3, 16 sources: central differences with a step of 1e-4 relative
+    (||eps d|| = 1e-4 ||m||) along 8 seeded unit directions."""
     torch = torch_cuda
     from psp_amd import workloads
     from psp_amd.adjoint import HelmholtzMisfit, misfit_and_gradient
     wl = workloads.config5()
     disc = wl.disc
-    n = disc.nx
-    dev = "cuda"
-    F = torch.from_numpy(wl.B).to(dev)
-    rec = torch.from_numpy(wl.receivers).to(dev)
-    opts = {"m": 20, "tol": 1e-10, "maxIts": 600, "precision": "fp64"}
-    # observed data from the true model
-    import psp_amd
-    ht = psp_amd.DistHelmholtz(disc)
-    ht.Initialize(1.0 / np.sqrt(wl.m_true))
-    Ut = F.clone()
-    ht.Solve(Ut, 20, 1e-10, 600)
-    data = Ut[rec].clone()
-    del ht, Ut
-    m = torch.from_numpy(wl.m).to(dev)
+    F = torch.from_numpy(wl.B).cuda()
+    rec = torch.from_numpy(wl.receivers).cuda()
+    opts = {"m": 20, "tol": 1e-12, "maxIts": 1000, "precision": "fp64"}
+    data = _survey_data(torch, wl, 1e-12)
+    m = torch.from_numpy(wl.m).cuda()
     loss, g = misfit_and_gradient(m, F, rec, data, disc, opts)
     print(f"[adjoint 80^3] loss {loss.item():.6e} |g| {g.norm().item():.3e}", flush=True)
 
@@ -98,18 +127,19 @@ def test_gradient_80cube_central_fd(torch_cuda):
         with torch.no_grad():
             return HelmholtzMisfit.apply(mv, F, rec, data, disc, opts).item()
 
-    xx, yy, zz, hgrid = workloads.grid_coords(n, n, n)
-    for seed in (1, 2):
-        r = np.random.default_rng([seed, 55])
-        c = r.uniform(0.3, 0.7, 3)
-        bump = np.exp(-((xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2) / (2 * 0.1 ** 2))
-        delta = torch.from_numpy(wl.m * bump.reshape(-1)).to(dev)
-        eps = 1e-4
-        fd = (L(m + eps * delta) - L(m - eps * delta)) / (2 * eps)
-        an = torch.dot(g, delta).item()
-        print(f"[adjoint 80^3] direction {seed}: fd {fd:.9e} adjoint {an:.9e} "
-              f"rel {abs(fd - an) / abs(fd):.2e}", flush=True)
-        assert abs(fd - an) <= 1e-4 * abs(fd)
+    eps = 1e-4 * torch.linalg.norm(m).item()
+    worst = 0.0
+    for j in range(8):
+        r = np.random.default_rng([workloads.SEED0 + 5, 3, j])
+        d = r.standard_normal(m.shape[0])
+        d = torch.from_numpy(d / np.linalg.norm(d)).cuda()
+        fd = (L(m + eps * d) - L(m - eps * d)) / (2 * eps)
+        an = torch.dot(g, d).item()
+        rel = abs(fd - an) / abs(fd)
+        worst = max(worst, rel)
+        print(f"[adjoint 80^3] direction {j}: fd {fd:.9e} adjoint {an:.9e} rel {rel:.2e}",
+              flush=True)
+    assert worst <= 1e-4
 
     opts32 = dict(opts, precision="fp32")
     _, g32 = misfit_and_gradient(m, F, rec, data, disc, opts32)
