@@ -51,8 +51,12 @@ enum { PSP_HIP_LDL_1D = 0, PSP_HIP_LDL_SELINV_1D = 1 };
 
 /* Options for psp_hip_set_option. */
 enum {
-    PSP_HIP_OPT_SWEEP_GRAPH = 1 /* 1 (default): replay the sweep from a captured
-                                   hipGraph; 0: launch it kernel by kernel */
+    PSP_HIP_OPT_SWEEP_GRAPH = 1,     /* 1 (default): replay the sweep from a captured
+                                        hipGraph; 0: launch it kernel by kernel */
+    PSP_HIP_OPT_SOLVE_MIN_TILES = 2, /* front-solve workgroups wanted per launch when
+                                        choosing default tile shapes (default 256) */
+    PSP_HIP_OPT_SOLVE_MAX_SPLIT = 3  /* most inner slices per row block in the
+                                        default tile shapes (default 8) */
 };
 
 /* Flags for psp_hip_solve. */

@@ -11,7 +11,6 @@ namespace psp_amd {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMvCols = 256;  // inner indices per solve tile (must match the plan)
 
 // --------------------------------------------------------------------------
 // Complex arithmetic (interleaved re, im).
@@ -299,13 +298,14 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_gemm(const GemmDesc* descs, const GemmTile* tiles,
                                                     int tileStart, cx<T>* b0, int64_t st0,
                                                     cx<T>* b1, int64_t st1, cx<T>* b2,
-                                                    int64_t st2)
+                                                    int64_t st2, cx<T>* b3)
 {
     constexpr int TM = 64, TK = 16;
     const GemmTile tl = tiles[tileStart + blockIdx.x];
     const GemmDesc g = descs[tl.desc];
     const int bt = blockIdx.y;
-    cx<T>* bases[3] = {b0 + (int64_t)bt * st0, b1 + (int64_t)bt * st1, b2 + (int64_t)bt * st2};
+    cx<T>* bases[4] = {b0 + (int64_t)bt * st0, b1 + (int64_t)bt * st1, b2 + (int64_t)bt * st2,
+                       b3 + (int64_t)bt * st0};  // b3 (LT) has L's stride
     const cx<T>* A = bases[g.abuf] + g.a;
     const cx<T>* B = bases[g.bbuf] + g.b;
     cx<T>* Cm = bases[g.cbuf] + g.c;
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm(const GemmDesc* descs, const 
             const int i = tm + tx + 16 * p, j = tn + ty + 16 * q;
             if (i >= g.M || j >= g.N) continue;
             if ((f & G_CLOWER) && i < j) continue;
-            cx<T>* c = Cm + i + (int64_t)j * g.ldc;
+            cx<T>* c = (f & G_CTRANS) ? Cm + j + (int64_t)i * g.ldc : Cm + i + (int64_t)j * g.ldc;
             const cx<T> v = (f & G_NEG) ? -acc[p][q] : acc[p][q];
             *c = (f & G_BETA0) ? v : (*c + v);
         }
@@ -425,16 +425,17 @@ __global__ __launch_bounds__(kThreads) void k_inv_diag(const InvItem* items, cx<
 }
 
 // --------------------------------------------------------------------------
-// Front solves on the padded panel vector Xp (rows x k, interleaved).
-// Per node: forward  F1 W = [Xp_own; 0] + children's update vectors,
-//                    F2 y = Linv W_top            -> Xp_own,
-//                    F3 W_bot -= L_BL y;
-//           backward B1 W_top = D^-1 y - L_BL^T x_struct,
-//                    B2 x = Linv^T W_top          -> Xp_own.
-// The products run as split-K tiles; the last tile of a row block (an
-// atomic counter, reset by that tile) sums the partials in slot order.
+// Front solves on the padded panel vector Xp (rows x k, interleaved), with
+// the factors in the solve layout (rows < s of L: Linv with a stored unit
+// diagonal; rows >= s: -L_BL Linv; LT: [Linv^T D^-1 | (-L_BL Linv)^T]).
+// Per node, with g = [Xp_own; 0] + the children's update vectors (gathered
+// through ginv), W the node's work rows (s + u):
+//   forward  F: W_top = Linv g_top,  W_bot = g_bot + (-L_BL Linv) g_top;
+//   backward B: Xp_own = LT [W_top; x_struct].
+// Each is one product over the node's factor block, split into row tiles
+// (and, near the root, inner slices summed by a reduce launch).
 // --------------------------------------------------------------------------
-enum { PH_F2 = 0, PH_F3 = 1, PH_B1 = 2, PH_B2 = 3 };
+enum { PH_F = 0, PH_B = 1 };
 
 // Forward input of front row r: the panel right-hand side on the node's own
 // rows plus the children's update vectors (extend-add through ginv).
@@ -450,55 +451,80 @@ __device__ __forceinline__ cx<T> Gathered(const NodeDev& nd, const int32_t* ginv
     return v;
 }
 
-// F2: W_top = y; F3: W_bot = gathered - sum; B1: W_top = y/d - sum; B2: Xp = x.
+// Inner operand of a phase at inner index col, right-hand side rr.
 template <typename T, int PH>
-__device__ __forceinline__ void MvFinal(const NodeDev& nd, const int32_t* ginv, const cx<T>* Lf,
-                                        cx<T>* Xp, cx<T>* W, int row, int r, int k, cx<T> a)
+__device__ __forceinline__ cx<T> InnerOperand(const NodeDev& nd, const int32_t* structs,
+                                              const int32_t* ginv, const cx<T>* Xp,
+                                              const cx<T>* W, int col, int rr, int k)
 {
-    cx<T>* Wt = W + nd.w * k;
-    if (PH == PH_F2) {
-        Wt[(int64_t)row * k + r] = a;
-    } else if (PH == PH_F3) {
-        Wt[(int64_t)(nd.s + row) * k + r] = Gathered(nd, ginv, Xp, W, nd.s + row, r, k) - a;
-    } else if (PH == PH_B1) {
-        const int fs = nd.s + nd.u;
-        const cx<T> dinv = cinv(Lf[row + (int64_t)row * fs]);
-        cx<T>* p = Wt + (int64_t)row * k + r;
-        *p = *p * dinv - a;
-    } else {
+    if (PH == PH_F) return Gathered(nd, ginv, Xp, W, col, rr, k);
+    if (col < nd.s) return W[(nd.w + col) * k + rr];
+    return Xp[(int64_t)structs[nd.ustart + col - nd.s] * k + rr];
+}
+
+// Factor entry of phase PH at (row, col) of the node's block.
+template <typename T, int PH>
+__device__ __forceinline__ cx<T> FactorEntry(const cx<T>* Lf, const cx<T>* Lt, int s, int fs,
+                                             int row, int col)
+{
+    if (PH == PH_F) return col <= row ? Lf[row + (int64_t)col * fs] : czero<T>();
+    return col >= row ? Lt[row + (int64_t)col * s] : czero<T>();
+}
+
+// Finish of output row `row`, right-hand side r, product a: F writes W
+// (top rows: y; bottom rows: gathered + a), B writes the node's Xp rows.
+// For F bottom rows `g` is the gathered value.
+template <typename T, int PH>
+__device__ __forceinline__ void Finish(const NodeDev& nd, cx<T>* Xp, cx<T>* W, int row, int r,
+                                       int k, cx<T> a, cx<T> g)
+{
+    if (PH == PH_F)
+        W[(nd.w + row) * k + r] = row < nd.s ? a : g + a;
+    else
         Xp[(int64_t)(nd.off + row) * k + r] = a;
-    }
+}
+
+template <typename T, int PH>
+__device__ __forceinline__ void FinishGather(const NodeDev& nd, const int32_t* ginv, cx<T>* Xp,
+                                             cx<T>* W, int row, int r, int k, cx<T> a)
+{
+    const cx<T> g = (PH == PH_F && row >= nd.s) ? Gathered(nd, ginv, Xp, W, row, r, k)
+                                                : czero<T>();
+    Finish<T, PH>(nd, Xp, W, row, r, k, a, g);
 }
 
 constexpr int kMvThreads = 512;  // 8 waves split each row block's inner indices
 
-// One workgroup per (node, block of RT output rows); the inner index range
-// streams through LDS in chunks of kMvCols.  Lane l of a wave owns row
-// l % RT and column slot l / RT (64 / RT columns per wave step); the slots
-// are combined by lane shuffles and the 8 waves' partial sums by a fixed-
-// order tree in LDS, so every output element is summed in a fixed order.
+// Vector-unit path (k < 8).  One workgroup per tile (a block of RT output
+// rows and an inner range).  Lane l of a wave owns row l % RT and column
+// slot l / RT (64 / RT columns per wave step); the inner range is cut into
+// chunks of NI columns per lane.  For each chunk the lane first issues all
+// NI of its factor loads, then the workgroup stages the matching vector
+// rows in LDS, so the factor and vector reads of a chunk are in flight
+// together.  Column slots are combined by lane shuffles and the 8 waves'
+// partial sums by a fixed-order tree in LDS: every output element is summed
+// in a fixed order.
 template <typename T, int PH, int RT>
-__global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int start,
-                                                    const NodeDev* nodes, const int32_t* structs,
+__global__ __launch_bounds__(kMvThreads) void k_mv(const SolveTile* tiles, int start,
+                                                    const int32_t* structs,
                                                     const int32_t* ginv, const cx<T>* L,
-                                                    const cx<T>* LT, cx<T>* Xp, cx<T>* W, int k)
+                                                    const cx<T>* LT, cx<T>* Xp, cx<T>* W,
+                                                    cx<T>* P, int k)
 {
-    constexpr int KC = 8, CC = kMvCols, NW = kMvThreads / 64, U = 16, CPW = 64 / RT;
-    const MvTile tl = tiles[start + blockIdx.x];
-    const NodeDev nd = nodes[tl.node];
+    constexpr int KC = 8, NW = kMvThreads / 64, CPW = 64 / RT, STRIDE = CPW * NW;
+    constexpr int NI = RT == 16 ? 16 : 32, CC = STRIDE * NI;
+    const SolveTile tl = tiles[start + blockIdx.x];
+    const NodeDev nd = tl.nd;
     const int s = nd.s, fs = nd.s + nd.u;
     const cx<T>* Lf = L + nd.l;
     const cx<T>* Lt = LT + nd.l;
-    const cx<T>* Wt = W + nd.w * k;
-    __shared__ cx<T> vs[CC][KC];  // staged vector chunk; reused for the wave reduction
-    static_assert(CC * KC >= (NW / 2) * RT * KC, "reduction scratch must fit in vs");
-    cx<T>(*red)[RT][KC] = reinterpret_cast<cx<T>(*)[RT][KC]>(&vs[0][0]);
+    __shared__ cx<T> vs[CC][KC];            // staged vector chunk
+    __shared__ cx<T> red[NW / 2][RT][KC];   // wave partial sums
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int rl = lane % RT, slot = lane / RT;
     const int row = tl.row0 + rl;
     const bool rowOk = rl < tl.nrows;
     const int first = slot + CPW * wave;  // this lane's first column in a chunk
-    constexpr int STRIDE = CPW * NW;      // and its column stride
     for (int r0 = 0; r0 < k; r0 += KC) {
         const int kc = min(KC, k - r0);
         cx<T> acc[KC];
@@ -506,50 +532,26 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int star
         for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
         for (int cb = tl.col0; cb < tl.col1; cb += CC) {
             const int ncol = min(CC, tl.col1 - cb);
-            __syncthreads();
+            cx<T> m[NI];
+#pragma unroll
+            for (int u = 0; u < NI; ++u) {
+                const int c = first + STRIDE * u;
+                m[u] = (rowOk && c < ncol) ? FactorEntry<T, PH>(Lf, Lt, s, fs, row, cb + c)
+                                           : czero<T>();
+            }
+            __syncthreads();  // the previous chunk's readers are done with vs
             for (int e = tid; e < ncol * KC; e += kMvThreads) {
                 const int c = e / KC, r = e % KC;
-                cx<T> v = czero<T>();
-                if (r < kc) {
-                    const int col = cb + c;
-                    if (PH == PH_F2)
-                        v = Gathered(nd, ginv, Xp, W, col, r0 + r, k);
-                    else if (PH == PH_F3 || PH == PH_B2)
-                        v = Wt[(int64_t)col * k + r0 + r];
-                    else
-                        v = Xp[(int64_t)structs[nd.ustart + col] * k + r0 + r];
-                }
-                vs[c][r] = v;
+                vs[c][r] = r < kc ? InnerOperand<T, PH>(nd, structs, ginv, Xp, W, cb + c, r0 + r, k)
+                                  : czero<T>();
             }
             __syncthreads();
-            const int nIter = ncol > first ? (ncol - first + STRIDE - 1) / STRIDE : 0;
-            for (int i0 = 0; i0 < nIter; i0 += U) {
-                cx<T> m[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int col = cb + first + STRIDE * (i0 + u);
-                    cx<T> v = czero<T>();
-                    if (rowOk && i0 + u < nIter) {
-                        if (PH == PH_F2)
-                            v = col < row ? Lf[row + (int64_t)col * fs]
-                                          : (col == row ? cx<T>{T(1), T(0)} : czero<T>());
-                        else if (PH == PH_F3)
-                            v = Lf[s + row + (int64_t)col * fs];
-                        else if (PH == PH_B1)
-                            v = Lt[row + (int64_t)(s + col) * s];
-                        else
-                            v = col > row ? Lt[row + (int64_t)col * s]
-                                          : (col == row ? cx<T>{T(1), T(0)} : czero<T>());
-                    }
-                    m[u] = v;
-                }
+            for (int u = 0; u < NI; ++u) {
+                const int c = first + STRIDE * u;
+                if (c < ncol) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (i0 + u < nIter) {
-                        const int c = first + STRIDE * (i0 + u);
-#pragma unroll
-                        for (int r = 0; r < KC; ++r) acc[r] = cfma(m[u], vs[c][r], acc[r]);
-                    }
+                    for (int r = 0; r < KC; ++r) acc[r] = cfma(m[u], vs[c][r], acc[r]);
                 }
             }
         }
@@ -563,7 +565,6 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int star
             }
         }
         for (int h = NW / 2; h >= 1; h >>= 1) {
-            __syncthreads();
             if (wave >= h && wave < 2 * h && slot == 0) {
 #pragma unroll
                 for (int r = 0; r < KC; ++r) red[wave - h][rl][r] = acc[r];
@@ -573,21 +574,231 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const MvTile* tiles, int star
 #pragma unroll
                 for (int r = 0; r < KC; ++r) acc[r] = acc[r] + red[wave][rl][r];
             }
+            __syncthreads();
         }
         if (wave == 0 && slot == 0 && rowOk) {
 #pragma unroll
             for (int r = 0; r < KC; ++r)
-                if (r < kc) MvFinal<T, PH>(nd, ginv, Lf, Xp, W, row, r0 + r, k, acc[r]);
+                if (r < kc) {
+                    if (tl.slot >= 0)
+                        P[((int64_t)tl.slot * 64 + rl) * k + r0 + r] = acc[r];
+                    else
+                        FinishGather<T, PH>(nd, ginv, Xp, W, row, r0 + r, k, acc[r]);
+                }
         }
-        __syncthreads();
     }
 }
 
-// LT = L^T per front, through a 32x32 LDS tile.
+// --------------------------------------------------------------------------
+// Front-solve products on the matrix cores (k >= 8 right-hand sides).
+//
+// One 16x16x4 real MFMA computes D(16x16) += A(16x4) B(4x16).  A complex
+// product out = M v (M: rows x inner, v: inner x k) maps onto two of them by
+// splitting the 16 output columns into the real parts (j < 8) and imaginary
+// parts (j >= 8) of 8 right-hand sides:
+//     D += Re(M) [Re v | Im v]  +  Im(M) [-Im v | Re v],
+// which is exactly 8 real flops per complex multiply-add, all on the MFMA
+// units (no 3M recombination).  Lane l of a wave holds A(l % 16, l / 16) and
+// B(l / 16, l % 16); its accumulator register q holds D(row, l % 16) with
+// row = 4 (l / 16) + q for the f32 instruction and l / 16 + 4 q for f64.
+// --------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_transpose(const TransItem* items,
-                                                         const NodeDev* nodes, const cx<T>* L,
-                                                         cx<T>* LT, int64_t fstore)
+struct Mfma;
+template <>
+struct Mfma<float> {
+    typedef v4f acc;
+    static __device__ __forceinline__ acc step(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
+};
+template <>
+struct Mfma<double> {
+    typedef v4d acc;
+    static __device__ __forceinline__ acc step(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int q) { return lk + 4 * q; }
+};
+
+constexpr int kMmThreads = 256;  // 4 waves
+
+// Matrix-core path (k >= 8).  One workgroup per tile; wave w owns rows
+// 16 (w / WK) .. +16 of the tile and every WK-th 4-column step of its inner
+// range.  The inner range streams through LDS in chunks of CC columns (all
+// KB right-hand sides of a pass); the factor loads of the next U steps are
+// issued before the current ones are consumed.  Waves that share rows sum
+// their partial tiles in a fixed order through LDS.  Loads that do not
+// depend on the products (the first factor batch, the staged vector, and
+// for single-pass blocks the gathered bottom rows of F) are issued right
+// after the tile record arrives, so a workgroup waits for few memory round
+// trips.
+template <typename T, int PH, int RT, int KB>
+__global__ __launch_bounds__(kMmThreads) void k_mvm(const SolveTile* tiles, int start,
+                                                     const int32_t* structs,
+                                                     const int32_t* ginv, const cx<T>* L,
+                                                     const cx<T>* LT, cx<T>* Xp, cx<T>* W,
+                                                     cx<T>* P, int k)
+{
+    typedef typename Mfma<T>::acc acc_t;
+    constexpr int NWV = kMmThreads / 64, WR = RT / 16, WK = NWV / WR, NB = KB / 8;
+    constexpr int CC = 2048 / KB, U = sizeof(T) == 4 ? 16 : 8;  // 8 KB of factor loads per wave
+    constexpr bool PRE = KB <= 16 && PH == PH_F;
+    static_assert(WR * WK == NWV && CC % 4 == 0, "tile shape");
+    static_assert(WK == 1 || 2 * CC * KB >= WR * NB * 256, "reduction scratch must fit in vs");
+    const SolveTile tl = tiles[start + blockIdx.x];
+    const NodeDev nd = tl.nd;
+    const int s = nd.s, fs = nd.s + nd.u;
+    const cx<T>* Lf = L + nd.l;
+    const cx<T>* Lt = LT + nd.l;
+    __shared__ cx<T> vs[CC][KB];
+    T* red = reinterpret_cast<T*>(&vs[0][0]);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave / WK, wk = wave % WK;
+    const int li = lane & 15, lk = lane >> 4;
+    const int arow = tl.row0 + 16 * wr + li;  // row of this lane's A entries
+    const bool arowOk = 16 * wr + li < tl.nrows;
+    // Finishing lanes: wk == 0, li < 8; element (b, q) is row
+    // 16 wr + row(lk, q) of the tile and right-hand side 8 b + li of the pass.
+    const bool fin = wk == 0 && li < 8 && tl.slot < 0;
+    cx<T> pre[PRE ? NB : 1][4];
+    if (PRE && fin && k <= KB) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = tl.row0 + 16 * wr + Mfma<T>::row(lk, q), rr = 8 * b + li;
+                pre[PRE ? b : 0][q] = (row < tl.row0 + tl.nrows && row >= s && rr < k)
+                                          ? Gathered(nd, ginv, Xp, W, row, rr, k)
+                                          : czero<T>();
+            }
+    }
+    for (int r0 = 0; r0 < k; r0 += KB) {
+        const int kc = min(KB, k - r0);
+        acc_t acc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = acc_t{0, 0, 0, 0};
+        for (int cb = tl.col0; cb < tl.col1; cb += CC) {
+            const int ncol = min(CC, tl.col1 - cb);
+            const int nsteps = (ncol + 3) >> 2;
+            const int mine = nsteps > wk ? (nsteps - wk + WK - 1) / WK : 0;
+            cx<T> m[U];
+            auto load = [&](cx<T>* dst, int u0) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int c = 4 * (wk + WK * (u0 + u)) + lk;
+                    dst[u] = (arowOk && u0 + u < mine && c < ncol)
+                                 ? FactorEntry<T, PH>(Lf, Lt, s, fs, arow, cb + c)
+                                 : czero<T>();
+                }
+            };
+            load(m, 0);
+            __syncthreads();  // the previous chunk's (or pass's) readers are done with vs
+            const int nst = min(CC, 4 * nsteps);
+            for (int e = tid; e < nst * KB; e += kMmThreads) {
+                const int c = e / KB, r = e % KB;
+                vs[c][r] = (r < kc && c < ncol)
+                               ? InnerOperand<T, PH>(nd, structs, ginv, Xp, W, cb + c, r0 + r, k)
+                               : czero<T>();
+            }
+            __syncthreads();
+            for (int u0 = 0; u0 < mine; u0 += U) {
+                cx<T> mn[U];
+                if (u0 + U < mine) load(mn, u0 + U);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (u0 + u < mine) {
+                        const int c = 4 * (wk + WK * (u0 + u)) + lk;
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) {
+                            const cx<T> v = vs[c][8 * b + (li & 7)];
+                            const T b1 = li < 8 ? v.r : v.i;
+                            const T b2 = li < 8 ? -v.i : v.r;
+                            acc[b] = Mfma<T>::step(m[u].r, b1, acc[b]);
+                            acc[b] = Mfma<T>::step(m[u].i, b2, acc[b]);
+                        }
+                    }
+                }
+                if (u0 + U < mine) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) m[u] = mn[u];
+                }
+            }
+        }
+        // waves wk = 1 .. WK-1 hand their partial tiles to wave wk = 0, in order
+        if (WK > 1) {
+            for (int w = 1; w < WK; ++w) {
+                __syncthreads();
+                if (wk == w) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) red[((wr * NB + b) * 64 + lane) * 4 + q] = acc[b][q];
+                }
+                __syncthreads();
+                if (wk == 0) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            acc[b][q] += red[((wr * NB + b) * 64 + lane) * 4 + q];
+                }
+            }
+        }
+        if (wk == 0) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const T val = acc[b][q];
+                    const T other = __shfl_xor(val, 8, 64);
+                    const int rl = 16 * wr + Mfma<T>::row(lk, q);  // row within the tile
+                    const int rr = 8 * b + li;                      // right-hand side within the pass
+                    if (li < 8 && rl < tl.nrows && rr < kc) {
+                        const cx<T> a{val, other};
+                        const int row = tl.row0 + rl;
+                        if (tl.slot >= 0)
+                            P[((int64_t)tl.slot * 64 + rl) * k + r0 + rr] = a;
+                        else if (PRE && k <= KB)
+                            Finish<T, PH>(nd, Xp, W, row, rr, k, a, pre[PRE ? b : 0][q]);
+                        else
+                            FinishGather<T, PH>(nd, ginv, Xp, W, row, r0 + rr, k, a);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Sum of the inner slices of split row blocks, in slot order, then the
+// phase's finish.  One workgroup per row block.
+template <typename T, int PH>
+__global__ __launch_bounds__(kThreads) void k_mv_reduce(const SolveTile* items, int start,
+                                                        const int32_t* ginv, cx<T>* Xp,
+                                                        cx<T>* W, const cx<T>* P, int k)
+{
+    const SolveTile it = items[start + blockIdx.x];
+    const NodeDev nd = it.nd;
+    for (int e = threadIdx.x; e < it.nrows * k; e += kThreads) {
+        const int rl = e / k, r = e % k;
+        cx<T> a = czero<T>();
+        for (int j = 0; j < it.col1; ++j) a = a + P[((int64_t)(it.col0 + j) * 64 + rl) * k + r];
+        FinishGather<T, PH>(nd, ginv, Xp, W, it.row0 + rl, r, k, a);
+    }
+}
+
+// LT[:, :s] = Linv^T D^-1 per front (upper triangle; D^-1 on the diagonal),
+// through a 32x32 LDS tile of L_TL (strictly lower part: Linv; diagonal: D).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_transpose_scale(const TransItem* items,
+                                                               const NodeDev* nodes,
+                                                               const cx<T>* L, cx<T>* LT,
+                                                               int64_t fstore)
 {
     const TransItem it = items[blockIdx.x];
     const NodeDev nd = nodes[it.node];
@@ -595,15 +806,51 @@ __global__ __launch_bounds__(kThreads) void k_transpose(const TransItem* items,
     const cx<T>* Lf = L + (int64_t)blockIdx.y * fstore + nd.l;
     cx<T>* Lt = LT + (int64_t)blockIdx.y * fstore + nd.l;
     __shared__ cx<T> tile[32][33];
+    __shared__ cx<T> dinv[32];
     const int t32 = threadIdx.x & 31, q = threadIdx.x >> 5;
+    if (threadIdx.x < 32) {
+        const int i = it.i0 + t32;
+        dinv[t32] = i < s ? cinv(Lf[i + (int64_t)i * fs]) : czero<T>();
+    }
     for (int jj = q; jj < 32; jj += 8) {
         const int i = it.i0 + t32, j = it.j0 + jj;
-        if (i < fs && j < s) tile[jj][t32] = Lf[i + (int64_t)j * fs];
+        if (i < s && j < s) tile[jj][t32] = Lf[i + (int64_t)j * fs];
     }
     __syncthreads();
     for (int il = q; il < 32; il += 8) {
         const int j = it.j0 + t32, i = it.i0 + il;
-        if (i < fs && j < s) Lt[j + (int64_t)i * s] = tile[t32][il];
+        if (i < s && j < s)
+            Lt[j + (int64_t)i * s] =
+                i > j ? tile[t32][il] * dinv[il] : (i == j ? dinv[il] : czero<T>());
+    }
+}
+
+// L_BL = (LT[:, s:])^T, i.e. -L_BL Linv back in column-major L, and the unit
+// diagonal of Linv stored in L (items with i0 == 0 of each column block).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_fold_back(const TransItem* items,
+                                                         const NodeDev* nodes, cx<T>* L,
+                                                         const cx<T>* LT, int64_t fstore)
+{
+    const TransItem it = items[blockIdx.x];
+    const NodeDev nd = nodes[it.node];
+    const int s = nd.s, u = nd.u, fs = nd.s + nd.u;
+    cx<T>* Lf = L + (int64_t)blockIdx.y * fstore + nd.l;
+    const cx<T>* Lt = LT + (int64_t)blockIdx.y * fstore + nd.l;
+    __shared__ cx<T> tile[32][33];
+    const int t32 = threadIdx.x & 31, q = threadIdx.x >> 5;
+    for (int il = q; il < 32; il += 8) {
+        const int i = it.i0 + il, j = it.j0 + t32;  // LT(j, s + i)
+        if (i < u && j < s) tile[il][t32] = Lt[j + (int64_t)(s + i) * s];
+    }
+    __syncthreads();
+    for (int jj = q; jj < 32; jj += 8) {
+        const int i = it.i0 + t32, j = it.j0 + jj;  // L(s + i, j)
+        if (i < u && j < s) Lf[s + i + (int64_t)j * fs] = tile[t32][jj];
+    }
+    if (it.i0 == 0 && threadIdx.x < 32) {
+        const int j = it.j0 + threadIdx.x;
+        if (j < s) Lf[j + (int64_t)j * fs] = cx<T>{T(1), T(0)};
     }
 }
 
@@ -940,7 +1187,7 @@ void LaunchGemm(const ClassDev& c, int tileStart, int tileCount, hipStream_t s)
     cx<T>* T0 = (cx<T>*)c.T;
     hipLaunchKernelGGL((k_gemm<T>), dim3(tileCount, c.npanels), dim3(kThreads), 0, s, c.gemms,
                        c.tiles, tileStart, (cx<T>*)c.L, c.fstore, (cx<T>*)c.U, c.ustore, T0,
-                       c.tstore);
+                       c.tstore, (cx<T>*)c.LT);
 }
 
 template <typename T>
@@ -953,26 +1200,49 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s)
 }
 
 template <typename T>
-void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, int k,
+void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, void* Pv, int k,
                       hipStream_t s, MvProfile* prof)
 {
     const cx<T>* L = (const cx<T>*)c.L + (int64_t)b * c.fstore;
     const cx<T>* LT = (const cx<T>*)c.LT + (int64_t)b * c.fstore;
     cx<T>* Xp = (cx<T>*)Xpv;
     cx<T>* W = (cx<T>*)Wv;
+    cx<T>* P = (cx<T>*)Pv;
     auto mv = [&](auto ph, const ClassPlan::SolveLevel& lv, const ClassPlan::Range& r) {
         if (r.count <= 0) return;
         constexpr int PH = decltype(ph)::value;
         if (prof) (void)hipEventRecord(prof->a, s);
-        if (r.rows == 64)
+        if (k >= 8) {
+            auto launch = [&](auto rt, auto kb) {
+                constexpr int RT = decltype(rt)::value, KB = decltype(kb)::value;
+                hipLaunchKernelGGL((k_mvm<T, PH, RT, KB>), dim3(r.count), dim3(kMmThreads), 0, s,
+                                   c.mv, r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+            };
+            auto byRows = [&](auto kb) {
+                if (r.rows == 64)
+                    launch(std::integral_constant<int, 64>(), kb);
+                else if (r.rows == 32)
+                    launch(std::integral_constant<int, 32>(), kb);
+                else
+                    launch(std::integral_constant<int, 16>(), kb);
+            };
+            if (k <= 8)
+                byRows(std::integral_constant<int, 8>());
+            else if (k <= 16)
+                byRows(std::integral_constant<int, 16>());
+            else if (k <= 32)
+                byRows(std::integral_constant<int, 32>());
+            else
+                byRows(std::integral_constant<int, 64>());
+        } else if (r.rows == 64)
             hipLaunchKernelGGL((k_mv<T, PH, 64>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
-                               r.start, c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
+                               r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
         else if (r.rows == 32)
             hipLaunchKernelGGL((k_mv<T, PH, 32>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
-                               r.start, c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
+                               r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
         else
             hipLaunchKernelGGL((k_mv<T, PH, 16>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
-                               r.start, c.nodes, c.structs, c.ginv, L, LT, Xp, W, k);
+                               r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
         if (prof) {
             (void)hipEventRecord(prof->b, s);
             (void)hipEventSynchronize(prof->b);
@@ -984,25 +1254,34 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
             prof->bytes += eb * (lv.entries[PH] + lv.vecRows[PH] * k);
             prof->flops += 8.0 * lv.entries[PH] * k;
         }
+        if (r.rcount > 0)
+            hipLaunchKernelGGL((k_mv_reduce<T, PH>), dim3(r.rcount), dim3(kThreads), 0, s,
+                               c.mvred, r.rstart, c.ginv, Xp, W, P, k);
     };
     for (int d = plan.maxDepth; d >= 0; --d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
-        mv(std::integral_constant<int, PH_F2>(), lv, lv.f2);
-        mv(std::integral_constant<int, PH_F3>(), lv, lv.f3);
+        mv(std::integral_constant<int, PH_F>(), lv, lv.f);
     }
     for (int d = 0; d <= plan.maxDepth; ++d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
-        mv(std::integral_constant<int, PH_B1>(), lv, lv.b1);
-        mv(std::integral_constant<int, PH_B2>(), lv, lv.b2);
+        mv(std::integral_constant<int, PH_B>(), lv, lv.b);
     }
 }
 
 template <typename T>
-void LaunchTranspose(const ClassDev& c, hipStream_t s)
+void LaunchTransposeScale(const ClassDev& c, hipStream_t s)
 {
     if (c.ntrans <= 0) return;
-    hipLaunchKernelGGL((k_transpose<T>), dim3(c.ntrans, c.npanels), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL((k_transpose_scale<T>), dim3(c.ntrans, c.npanels), dim3(kThreads), 0, s,
                        c.trans, c.nodes, (const cx<T>*)c.L, (cx<T>*)c.LT, c.fstore);
+}
+
+template <typename T>
+void LaunchFoldBack(const ClassDev& c, hipStream_t s)
+{
+    if (c.nfold <= 0) return;
+    hipLaunchKernelGGL((k_fold_back<T>), dim3(c.nfold, c.npanels), dim3(kThreads), 0, s,
+                       c.foldBack, c.nodes, (cx<T>*)c.L, (const cx<T>*)c.LT, c.fstore);
 }
 
 template <typename T>
@@ -1109,9 +1388,10 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
     template void LaunchLdlTrsm<T>(const ClassDev&, int, int, int, hipStream_t);                  \
     template void LaunchGemm<T>(const ClassDev&, int, int, hipStream_t);                          \
     template void LaunchInvDiag<T>(const ClassDev&, int, hipStream_t);                            \
-    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, int,  \
+    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, void*, int,  \
                                       hipStream_t, MvProfile*);                                   \
-    template void LaunchTranspose<T>(const ClassDev&, hipStream_t);                               \
+    template void LaunchTransposeScale<T>(const ClassDev&, hipStream_t);                          \
+    template void LaunchFoldBack<T>(const ClassDev&, hipStream_t);                                \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
                                   int, hipStream_t);                                              \
     template void LaunchCopyOut<T>(const ClassDev&, c128*, const void*, int, int, int, int, int,  \

@@ -39,6 +39,15 @@ struct NodeDev {
     int32_t ustart;    // into structs
 };
 
+// A front-solve tile (or reduce item) with its node's fields inlined, so a
+// workgroup starts with one 64-byte load.
+struct SolveTile {
+    NodeDev nd;
+    int32_t row0, nrows, col0, col1, slot;
+    int32_t pad_[3];
+};
+static_assert(sizeof(SolveTile) == 64, "SolveTile is one 64-byte record");
+
 // Per-class device copy of the static plan.
 struct ClassDev {
     int n = 0, vsize = 0, nnodes = 0, npanels = 0;
@@ -54,10 +63,12 @@ struct ClassDev {
     GemmTile* tiles = nullptr;
     InvItem* inv = nullptr;
     NodeDev* nodes = nullptr;
-    MvTile* mv = nullptr;
-    TransItem* trans = nullptr;
+    SolveTile* mv = nullptr;
+    SolveTile* mvred = nullptr;     // reduce items of split solve row blocks
+    TransItem* trans = nullptr;     // LT[:, :s] blocks
+    TransItem* foldBack = nullptr;  // L_BL / diagonal blocks
     int32_t* ginv = nullptr;
-    int ninv = 0, ntrans = 0;
+    int ninv = 0, ntrans = 0, nfold = 0;
     int32_t* panelVOff = nullptr;   // per batch entry: panel vOffset
     c128* s3art = nullptr;          // per batch entry: vsize + 2 profile values
     void* L = nullptr;              // npanels * fstore
@@ -96,9 +107,12 @@ struct MvProfile {
 // (n x k, k interleaved) with node work vectors W (wstore x k).
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xp, void* W,
+                      void* P,
                       int k, hipStream_t s, MvProfile* prof = nullptr);
 template <typename T>
-void LaunchTranspose(const ClassDev& c, hipStream_t s);
+void LaunchTransposeScale(const ClassDev& c, hipStream_t s);
+template <typename T>
+void LaunchFoldBack(const ClassDev& c, hipStream_t s);
 // Natural-order <-> padded panel vector.  mode 0: B = Xp, 1: B += Xp.
 template <typename T>
 void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int ny, int nz,

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "kernels.hpp"
@@ -49,6 +50,7 @@ struct psp_hip_ctx {
     int wsK = 0, wsM = 0;
     std::vector<void*> wsAllocs;
     void *Xp = nullptr, *W = nullptr;
+    void* P = nullptr;  // partial sums of split solve row blocks (64 rows x k per slot)
     c128 *V = nullptr, *x = nullptr, *x0 = nullptr, *w = nullptr, *b = nullptr;
     double *partD = nullptr, *normD = nullptr;
     c128 *partC = nullptr, *Hcol = nullptr, *ydev = nullptr;
@@ -165,6 +167,28 @@ StencilArgs Stencil(const psp_hip_ctx* c)
 size_t FrontBytes(const psp_hip_ctx* c) { return c->desc.precision == PSP_HIP_FRONTS_FP32 ? 8 : 16; }
 
 // One-time device setup: upload the static plan and allocate factor storage.
+NodeDev MakeNodeDev(const Node& n)
+{
+    return NodeDev{n.loff, n.woff, n.off, n.size, n.usize, (int32_t)n.ustart};
+}
+
+std::vector<SolveTile> SolveTiles(const ClassPlan& P, const std::vector<MvTile>& tiles)
+{
+    std::vector<SolveTile> out(tiles.size());
+    for (size_t i = 0; i < tiles.size(); ++i) {
+        const MvTile& m = tiles[i];
+        SolveTile& t = out[i];
+        t.nd = MakeNodeDev(P.nodes[m.node]);
+        t.row0 = m.row0;
+        t.nrows = m.nrows;
+        t.col0 = m.col0;
+        t.col1 = m.col1;
+        t.slot = m.slot;
+        t.pad_[0] = t.pad_[1] = t.pad_[2] = 0;
+    }
+    return out;
+}
+
 int EnsureDevice(psp_hip_ctx* c)
 {
     if (c->devReady) return PSP_HIP_OK;
@@ -255,17 +279,20 @@ int EnsureDevice(psp_hip_ctx* c)
             (st = Upload(c, &D.relinds, P.relinds)) || (st = Upload(c, &D.extadd, P.extadd)) ||
             (st = Upload(c, &D.ldl, P.ldl)) || (st = Upload(c, &D.gemms, P.gemms)) ||
             (st = Upload(c, &D.tiles, P.tiles)) || (st = Upload(c, &D.inv, P.inv)) ||
-            (st = Upload(c, &D.mv, P.mv)) ||
+            (st = Upload(c, &D.mv, SolveTiles(P, P.mv))) ||
+            (st = Upload(c, &D.mvred, SolveTiles(P, P.mvred))) ||
             (st = Upload(c, &D.trans, P.trans)) || (st = Upload(c, &D.ginv, P.ginv)) ||
+            (st = Upload(c, &D.foldBack, P.foldBack)) ||
             (st = Upload(c, &D.panelVOff, vOff)))
             return st;
         std::vector<NodeDev> nd(D.nnodes);
         for (int t = 0; t < D.nnodes; ++t) {
             const Node& n = P.nodes[t];
-            nd[t] = NodeDev{n.loff, n.woff, n.off, n.size, n.usize, (int32_t)n.ustart};
+            nd[t] = MakeNodeDev(n);
         }
         if ((st = Upload(c, &D.nodes, nd))) return st;
         D.ntrans = (int)P.trans.size();
+        D.nfold = (int)P.foldBack.size();
         std::vector<c128> s3c = ToC128(s3a);
         if ((st = Upload(c, &D.s3art, s3c))) return st;
         if ((st = Alloc(c, &D.L, eb * (size_t)D.npanels * D.fstore, c->allocs))) return st;
@@ -300,16 +327,18 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
     FreeWorkspace(c);
     const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
     const size_t eb = FrontBytes(c);
-    int64_t maxn = 0, maxw = 0;
+    int64_t maxn = 0, maxw = 0, maxs = 1;
     for (int cl = 0; cl < 3; ++cl)
         if (c->plan.present[cl]) {
             maxn = std::max<int64_t>(maxn, c->plan.cls[cl].n);
             maxw = std::max<int64_t>(maxw, c->plan.cls[cl].wstore);
+            maxs = std::max<int64_t>(maxs, c->plan.cls[cl].maxSlots);
         }
     int st;
     void* p;
     if ((st = Alloc(c, &c->Xp, eb * maxn * k, c->wsAllocs))) return st;
     if ((st = Alloc(c, &c->W, eb * maxw * k, c->wsAllocs))) return st;
+    if ((st = Alloc(c, &c->P, eb * maxs * 64 * k, c->wsAllocs))) return st;
     if ((st = Alloc(c, &p, sizeof(c128) * N * k * m, c->wsAllocs))) return st;
     c->V = (c128*)p;
     if ((st = Alloc(c, &p, sizeof(c128) * N * k, c->wsAllocs))) return st;
@@ -333,6 +362,40 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
     c->ydev = (c128*)p;
     c->wsK = k;
     c->wsM = m;
+    return PSP_HIP_OK;
+}
+
+// Replaces a device array uploaded by Upload with new contents.
+template <typename X>
+int ReUpload(psp_hip_ctx* c, X** dst, const std::vector<X>& src)
+{
+    if (*dst) {
+        auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)*dst);
+        if (it != c->allocs.end()) c->allocs.erase(it);
+        HIPCHK(c, hipFree(*dst));
+        *dst = nullptr;
+    }
+    return Upload(c, dst, src);
+}
+
+// Rebuilds the front-solve tiles of every class from its shapes and, once
+// the device is set up, replaces the device copies; the workspace (partial
+// buffer sized by the slots) and the captured sweep graphs are dropped.
+int RebuildSolveTiles(psp_hip_ctx* c)
+{
+    if (c->devReady) HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int cl = 0; cl < 3; ++cl) {
+        if (!c->plan.present[cl]) continue;
+        ClassPlan& P = c->plan.cls[cl];
+        BuildSolveTiles(P);
+        if (!c->devReady) continue;
+        ClassDev& D = c->dev[cl];
+        int st;
+        if ((st = ReUpload(c, &D.mv, SolveTiles(P, P.mv))) ||
+            (st = ReUpload(c, &D.mvred, SolveTiles(P, P.mvred))))
+            return st;
+    }
+    FreeWorkspace(c);
     return PSP_HIP_OK;
 }
 
@@ -368,7 +431,11 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv)
         HIPCHK(c, hipEventRecord(e[2], s));
         LaunchInvDiag<T>(D, nb, s);
         for (const GemmLaunch& gl : P.invGemm) LaunchGemm<T>(D, gl.tileStart, gl.tileCount, s);
-        LaunchTranspose<T>(D, s);
+        // solve layout: LT[:, :s] = Linv^T D^-1, LT[:, s:] = (-L_BL Linv)^T, then
+        // L_BL = -L_BL Linv and a unit diagonal in L
+        LaunchTransposeScale<T>(D, s);
+        LaunchGemm<T>(D, P.fold.tileStart, P.fold.tileCount, s);
+        LaunchFoldBack<T>(D, s);
         HIPCHK(c, hipEventRecord(e[3], s));
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventSynchronize(e[3]));
@@ -398,7 +465,7 @@ void PanelForward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = nullp
     const ClassDev& D = c->dev[cl];
     hipStream_t s = c->stream;
     LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s, prof);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, k, s, prof);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s);
 }
 
@@ -416,7 +483,7 @@ void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = null
     const int v2 = g.panelV(p) + g.depth(p) - 1;
     const int vlLast = g.padding(p) + g.depth(p) - 1;
     LaunchSupdiagSet<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, k, s, prof);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, k, s, prof);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 1, s);
 }
 
@@ -748,6 +815,21 @@ int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
         FreeGraphs(c);
         c->useGraph = value != 0;
         return PSP_HIP_OK;
+    }
+    if (option == PSP_HIP_OPT_SOLVE_MIN_TILES || option == PSP_HIP_OPT_SOLVE_MAX_SPLIT) {
+        if (option == PSP_HIP_OPT_SOLVE_MIN_TILES) {
+            if (value < 1 || value > (1 << 20))
+                return Fail(c, PSP_HIP_ERR_INVALID, "solve min tiles out of range");
+            c->plan.minTiles = value;
+        } else {
+            if (value < 1 || value > 64)
+                return Fail(c, PSP_HIP_ERR_INVALID, "solve max split out of range [1, 64]");
+            c->plan.maxSplit = value;
+        }
+        for (int cl = 0; cl < 3; ++cl)
+            if (c->plan.present[cl]) DefaultSolveShapes(c->plan.cls[cl], c->plan.minTiles,
+                                                        c->plan.maxSplit);
+        return RebuildSolveTiles(c);
     }
     return Fail(c, PSP_HIP_ERR_INVALID, "unknown option");
 }

@@ -519,7 +519,6 @@ void Schedule(const Plan& plan, ClassPlan& C)
 
     // Solve schedule.
     {
-        const int R = plan.mvRows;
         C.ginv.assign((size_t)2 * C.wstore, -1);
         for (const Node& n : C.nodes)
             for (int k = 0; k < 2; ++k) {
@@ -530,86 +529,41 @@ void Schedule(const Plan& plan, ClassPlan& C)
                     C.ginv[(size_t)2 * (n.woff + r) + k] = (int32_t)(ch.woff + ch.size + a);
                 }
             }
-        C.solveLevels.assign(C.maxDepth + 1, {});
-        // One tile per block of R output rows; for a block starting at r0 the
-        // inner range is [lo(r0), hi(r0)).  Per (level, phase) R is the
-        // largest of 64, 32, 16 that still gives kMinTiles workgroups, so
-        // the few big fronts near the root spread over more of the chip.
-        constexpr int kMinTiles = 256;
-        (void)R;
-        auto nrowsOf = [&](int ph, const Node& n) {
-            return (ph == 1) ? n.usize : n.size;
-        };
-        auto wanted = [&](int ph, const Node& n) {
-            if (ph == 1) return n.usize > 0;  // F3 also for empty nodes
-            return n.size > 0;
-        };
-        auto loOf = [&](int ph, const Node&, int r0) { return ph == 3 ? r0 : 0; };
-        auto hiOf = [&](int ph, const Node& n, int r0, int rows) {
-            if (ph == 0) return std::min(n.size, r0 + rows);
-            if (ph == 1) return n.size;
-            if (ph == 2) return n.usize;
-            return n.size;
-        };
-        auto build = [&](int ph, const std::vector<int>& lv, ClassPlan::Range& rg) {
-            int rows = 64;
-            for (int cand : {64, 32, 16}) {
-                int64_t tiles = 0;
-                for (int t : lv)
-                    if (wanted(ph, C.nodes[t])) tiles += CeilDiv(nrowsOf(ph, C.nodes[t]), cand);
-                rows = cand;
-                if (tiles >= kMinTiles) break;
-            }
-            rg.rows = rows;
-            rg.start = (int)C.mv.size();
-            for (int t : lv) {
-                const Node& n = C.nodes[t];
-                if (!wanted(ph, n)) continue;
-                const int nr = nrowsOf(ph, n);
-                for (int r0 = 0; r0 < nr; r0 += rows) {
-                    MvTile m;
-                    m.node = t;
-                    m.row0 = r0;
-                    m.nrows = std::min(rows, nr - r0);
-                    m.col0 = loOf(ph, n, r0);
-                    m.col1 = std::max(m.col0, hiOf(ph, n, r0, rows));
-                    m.pad_ = 0;
-                    C.mv.push_back(m);
-                }
-            }
-            rg.count = (int)C.mv.size() - rg.start;
-        };
-        auto account = [&](ClassPlan::SolveLevel& L, int ph, const ClassPlan::Range& r) {
-            for (int i = r.start; i < r.start + r.count; ++i) {
-                const MvTile& m = C.mv[i];
-                double e = 0;
-                for (int row = m.row0; row < m.row0 + m.nrows; ++row) {
-                    if (ph == 0) e += std::min(row, m.col1) - m.col0;        // strictly lower
-                    else if (ph == 3) e += m.col1 - std::max(row + 1, m.col0);  // strictly upper
-                    else e += m.col1 - m.col0;
-                }
-                L.entries[ph] += e;
-                L.vecRows[ph] += (m.col1 - m.col0) + m.nrows;
-            }
-        };
-        for (int d = 0; d <= C.maxDepth; ++d) {
-            ClassPlan::SolveLevel& L = C.solveLevels[d];
-            const std::vector<int>& lv = C.levels[d];
-            build(0, lv, L.f2);  // y = Linv W_top, lower triangular
-            build(1, lv, L.f3);  // W_bot = gathered - L_BL y (also for empty nodes)
-            build(2, lv, L.b1);  // W_top = D^-1 y - L_BL^T x_struct
-            build(3, lv, L.b2);  // Xp_own = Linv^T W_top, upper triangular
-            account(L, 0, L.f2);
-            account(L, 1, L.f3);
-            account(L, 2, L.b1);
-            account(L, 3, L.b2);
-        }
+        DefaultSolveShapes(C, plan.minTiles, plan.maxSplit);
+        BuildSolveTiles(C);
+        // Solve layout: LT[:, :s] = Linv^T D^-1 (32x32 blocks of L_TL),
+        // LT[:, s:] = (-L_BL Linv)^T by one batched GEMM, then L_BL and the
+        // diagonal of L from LT (an item with i0 == 0 per column block also
+        // stores the unit diagonal, so nodes without an update get one).
+        C.fold.tileStart = (int)C.tiles.size();
         for (int t = 0; t < nn; ++t) {
             const Node& n = C.nodes[t];
             const int fs = n.size + n.usize;
-            for (int i0 = 0; i0 < fs; i0 += 32)
+            for (int i0 = 0; i0 < n.size; i0 += 32)
                 for (int j0 = 0; j0 < n.size; j0 += 32) C.trans.push_back({t, i0, j0, 0});
+            for (int i0 = 0; i0 < std::max(n.usize, 1); i0 += 32)
+                for (int j0 = 0; j0 < n.size; j0 += 32) C.foldBack.push_back({t, i0, j0, 0});
+            if (n.size == 0 || n.usize == 0) continue;
+            GemmDesc g{};
+            g.a = n.loff + n.size;  // L_BL
+            g.lda = fs;
+            g.b = n.loff;           // Linv (unit lower)
+            g.ldb = fs;
+            g.c = n.loff + (int64_t)n.size * n.size;  // LT[:, s:], stored transposed
+            g.ldc = n.size;
+            g.d = 0;
+            g.dinc = 0;
+            g.M = n.usize;
+            g.N = n.size;
+            g.K = n.size;
+            g.abuf = BUF_L;
+            g.bbuf = BUF_L;
+            g.cbuf = BUF_LT;
+            g.dbuf = BUF_L;
+            g.flags = G_BUNIT | G_NEG | G_BETA0 | G_CTRANS;
+            AddGemmTiles(C, g, plan.tile);
         }
+        C.fold.tileCount = (int)C.tiles.size() - C.fold.tileStart;
     }
 
     // Algorithmic counts (8 real flops per complex multiply-add).
@@ -628,6 +582,147 @@ void Schedule(const Plan& plan, ClassPlan& C)
 }
 
 }  // namespace
+
+namespace {
+
+// Phase 0 (F): rows [0, s+u), inner [0, min(s, r0 + rows)) (rows < s are
+// lower triangular).  Phase 1 (B): rows [0, s), inner [r0, s+u) (upper
+// triangle, then the update columns).
+int SolveRowsOf(int ph, const Node& n) { return ph == 0 ? n.size + n.usize : n.size; }
+
+bool SolveWanted(int ph, const Node& n) { return ph == 0 ? n.size + n.usize > 0 : n.size > 0; }
+
+int SolveLo(int ph, int r0) { return ph == 1 ? r0 : 0; }
+int SolveHi(int ph, const Node& n, int r0, int rows)
+{
+    if (ph == 0) return std::min(n.size, r0 + rows);
+    return n.size + n.usize;
+}
+
+int64_t RowBlocks(const ClassPlan& C, int d, int ph, int rows)
+{
+    int64_t t = 0;
+    for (int v : C.levels[d])
+        if (SolveWanted(ph, C.nodes[v])) t += CeilDiv(SolveRowsOf(ph, C.nodes[v]), rows);
+    return t;
+}
+
+}  // namespace
+
+void DefaultSolveShapes(ClassPlan& C, int minTiles, int maxSplit)
+{
+    C.shapes.assign(C.maxDepth + 1, {});
+    for (int d = 0; d <= C.maxDepth; ++d)
+        for (int ph = 0; ph < 2; ++ph) {
+            SolveShape sh{16, 1};
+            for (int cand : {64, 32, 16}) {
+                sh.rows = cand;
+                if (RowBlocks(C, d, ph, cand) >= minTiles) break;
+            }
+            const int64_t t = RowBlocks(C, d, ph, sh.rows);
+            if (t > 0 && t < minTiles)
+                sh.split = (int)std::min<int64_t>(maxSplit, CeilDiv(minTiles, t));
+            C.shapes[d][ph] = sh;
+        }
+}
+
+void BuildSolveTiles(ClassPlan& C)
+{
+    C.mv.clear();
+    C.mvred.clear();
+    C.maxSlots = 0;
+    C.solveLevels.assign(C.maxDepth + 1, {});
+    // One tile per (block of `rows` output rows, inner slice).  A row block
+    // whose inner range [lo, hi) is long enough is cut into up to `split`
+    // slices of at least kMinSlice indices (multiples of 16); its tiles then
+    // write partial sums to consecutive slots, and one reduce item sums them
+    // in slot order.
+    constexpr int kMinSlice = 64;
+    auto build = [&](int d, int ph, ClassPlan::Range& rg) {
+        const SolveShape sh = C.shapes[d][ph];
+        rg.rows = sh.rows;
+        rg.start = (int)C.mv.size();
+        rg.rstart = (int)C.mvred.size();
+        int slots = 0;
+        for (int t : C.levels[d]) {
+            const Node& n = C.nodes[t];
+            if (!SolveWanted(ph, n)) continue;
+            const int nr = SolveRowsOf(ph, n);
+            for (int r0 = 0; r0 < nr; r0 += sh.rows) {
+                const int lo = SolveLo(ph, r0);
+                const int hi = std::max(lo, SolveHi(ph, n, r0, sh.rows));
+                const int len = hi - lo;
+                int per = len, nsl = 1;
+                if (sh.split > 1 && len >= 2 * kMinSlice) {
+                    per = std::max(kMinSlice, (int)CeilDiv(CeilDiv(len, sh.split), 16) * 16);
+                    nsl = (int)CeilDiv(len, per);
+                }
+                MvTile m;
+                m.node = t;
+                m.row0 = r0;
+                m.nrows = std::min(sh.rows, nr - r0);
+                if (nsl == 1) {
+                    m.col0 = lo;
+                    m.col1 = hi;
+                    m.slot = -1;
+                    C.mv.push_back(m);
+                    continue;
+                }
+                MvTile red = m;
+                red.col0 = slots;  // first slot
+                red.col1 = nsl;    // slot count
+                red.slot = 0;
+                C.mvred.push_back(red);
+                for (int j = 0; j < nsl; ++j) {
+                    m.col0 = lo + j * per;
+                    m.col1 = std::min(hi, lo + (j + 1) * per);
+                    m.slot = slots++;
+                    C.mv.push_back(m);
+                }
+            }
+        }
+        rg.count = (int)C.mv.size() - rg.start;
+        rg.rcount = (int)C.mvred.size() - rg.rstart;
+        C.maxSlots = std::max(C.maxSlots, slots);
+    };
+    // Compulsory work of a phase: the factor entries it applies, and per
+    // node its inner vector rows read once plus its output rows.
+    auto account = [&](ClassPlan::SolveLevel& L, int ph, const ClassPlan::Range& r) {
+        int node = -1, lo = 0, hi = 0, rows = 0, lastRow = -1;
+        auto flush = [&]() {
+            if (node >= 0) L.vecRows[ph] += (hi - lo) + rows;
+        };
+        for (int i = r.start; i < r.start + r.count; ++i) {
+            const MvTile& m = C.mv[i];
+            double e = 0;
+            for (int row = m.row0; row < m.row0 + m.nrows; ++row) {
+                if (ph == 0) e += std::max(0, std::min(row + 1, m.col1) - m.col0);  // lower incl. diagonal
+                else e += std::max(0, m.col1 - std::max(row, m.col0));             // upper incl. diagonal
+            }
+            L.entries[ph] += e;
+            if (m.node != node) {
+                flush();
+                node = m.node;
+                lo = m.col0;
+                hi = m.col1;
+                rows = 0;
+                lastRow = -1;
+            }
+            lo = std::min(lo, m.col0);
+            hi = std::max(hi, m.col1);
+            if (m.row0 != lastRow) rows += m.nrows;
+            lastRow = m.row0;
+        }
+        flush();
+    };
+    for (int d = 0; d <= C.maxDepth; ++d) {
+        ClassPlan::SolveLevel& L = C.solveLevels[d];
+        build(d, 0, L.f);  // [y; W_bot] = L g_top + [0; g_bot]
+        build(d, 1, L.b);  // x = LT [y; x_struct]
+        account(L, 0, L.f);
+        account(L, 1, L.b);
+    }
+}
 
 int BuildPlan(const psp_hip_desc& d, Plan& plan, std::string& msg)
 {

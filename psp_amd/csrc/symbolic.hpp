@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "psp_hip.h"
@@ -44,7 +45,7 @@ int NestedIndex(int x, int y, int vLocal, int xSize, int ySize, int vSize,
 // Device work-list records.  All offsets are element offsets inside one
 // panel's slice of the named buffer; the kernels add batch * stride.
 // ---------------------------------------------------------------------------
-enum BufId : int16_t { BUF_L = 0, BUF_U = 1, BUF_T = 2 };
+enum BufId : int16_t { BUF_L = 0, BUF_U = 1, BUF_T = 2, BUF_LT = 3 };
 
 enum GemmFlags : int32_t {
     G_BTRANS = 1,   // B(k,n) = Bp[n + k*ldb]
@@ -54,6 +55,7 @@ enum GemmFlags : int32_t {
     G_HASD = 16,    // multiply by diag(D) between A and B
     G_NEG = 32,     // C -= ...
     G_BETA0 = 64,   // C is overwritten instead of accumulated
+    G_CTRANS = 128, // C(i,j) is stored at Cp[j + i*ldc]
 };
 
 struct GemmDesc {
@@ -102,14 +104,24 @@ struct InvItem {
 
 // One row block of a matrix-vector product of one node in a front solve:
 // output rows [row0, row0 + nrows) (nrows <= 64), inner indices [col0, col1).
+// slot >= 0: the tile is one inner slice of its row block and writes its
+// partial sums to that slot of the partial buffer.  As a reduce item (same
+// struct): col0 = first slot, col1 = number of slots.
 struct MvTile {
     int32_t node;
     int32_t row0, nrows;
     int32_t col0, col1;
-    int32_t pad_;
+    int32_t slot;
 };
 
-// 32x32 block of the front transpose LT = L^T (s x (s+u), ld s).
+// Shape of the tiles of one (level, phase) of the front solves.
+struct SolveShape {
+    int rows = 64;  // output rows per tile: 64, 32 or 16
+    int split = 1;  // inner slices per row block (partial sums + reduce)
+};
+
+// 32x32 block (rows i0.., columns j0..) of a per-front copy between L and
+// LT (see the solve layout in ClassPlan).
 struct TransItem {
     int32_t node, i0, j0, pad_;
 };
@@ -163,22 +175,36 @@ struct ClassPlan {
     std::vector<InvItem> inv;
     std::vector<GemmLaunch> invGemm;      // alternating T = C*Ainv, X = -Binv*T
 
-    // Solve schedule.  Per depth: forward Linv (F2, reading the extend-added
-    // right-hand side), forward update (F3), backward update (B1), backward
-    // Linv^T (B2).
+    // Solve layout of the factors, made after the selective inversion:
+    //   L  ((s+u) x s, ld s+u): rows < s hold Linv (unit lower, ones stored
+    //      on the diagonal), rows >= s hold -L_BL Linv;
+    //   LT (s x (s+u), ld s):   columns < s hold Linv^T D^-1 (upper),
+    //      columns >= s hold (-L_BL Linv)^T.
+    // The forward solve of a node is then one product [y; W_bot] =
+    // L g_top + [0; g_bot] and the backward solve one product
+    // x = LT [y; x_struct]: one launch per level and direction.
+    GemmLaunch fold;                 // LT[:, s:] = (-L_BL Linv)^T
+    std::vector<TransItem> foldBack; // L_BL <- LT[:, s:]^T, diag(L) <- 1
+
+    // Solve schedule.  Per depth: forward product (F) and backward product
+    // (B).
     // ginv[2*r + c]: row of child c's work vector feeding front row r (or -1).
     std::vector<int32_t> ginv;
     std::vector<MvTile> mv;
+    std::vector<MvTile> mvred;  // reduce items of split row blocks
+    int maxSlots = 0;           // partial slots of the largest (level, phase)
+    std::vector<std::array<SolveShape, 2>> shapes;  // per depth, per phase (F, B)
     struct Range {
-        int start = 0, count = 0;
-        int rows = 64;  // output rows per tile (64, 32 or 16)
+        int start = 0, count = 0;    // tiles
+        int rows = 64;               // output rows per tile (64, 32 or 16)
+        int rstart = 0, rcount = 0;  // reduce items
     };
     struct SolveLevel {
-        Range f2, f3, b1, b2;
-        // per phase (F2, F3, B1, B2): factor entries read and vector rows
-        // (inner inputs + outputs) moved, for algorithmic bytes and flops
-        double entries[4] = {0, 0, 0, 0};
-        double vecRows[4] = {0, 0, 0, 0};
+        Range f, b;
+        // per phase (F, B): factor entries read and vector rows (inner
+        // inputs + outputs) moved, for algorithmic bytes and flops
+        double entries[2] = {0, 0};
+        double vecRows[2] = {0, 0};
     };
     std::vector<SolveLevel> solveLevels;  // index = depth
     std::vector<TransItem> trans;
@@ -194,10 +220,19 @@ struct Plan {
     bool present[3] = {false, false, false};
     int blockNb = 32;      // LDL^T block width
     int tile = 64;         // GEMM tile
-    int mvRows = 64;       // output rows per solve tile
+    int minTiles = 256;    // solve tiles wanted per launch (default shapes)
+    int maxSplit = 8;      // inner slices per row block at most (default shapes)
 };
 
 // Builds everything; returns PSP_HIP_OK or an error code with `msg` set.
 int BuildPlan(const psp_hip_desc& desc, Plan& plan, std::string& msg);
+
+// Solve tile shapes: the default per (level, phase) is the largest row
+// block of 64, 32, 16 rows that gives minTiles tiles, and otherwise inner
+// slices (at most maxSplit) to reach minTiles.
+void DefaultSolveShapes(ClassPlan& C, int minTiles, int maxSplit);
+// (Re)builds the solve tiles, reduce items and their accounting from
+// C.shapes.
+void BuildSolveTiles(ClassPlan& C);
 
 }  // namespace psp_amd

@@ -192,6 +192,16 @@ class DistHelmholtz:
         if st != _lib.OK:
             _raise(st, self._h, "set_option")
 
+    def set_solve_tiling(self, min_tiles=None, max_split=None):
+        """Default front-solve tile shapes: workgroups wanted per launch and
+        the most inner slices per row block (rebuilds the solve tiles)."""
+        for opt, v in ((_lib.OPT_SOLVE_MIN_TILES, min_tiles), (_lib.OPT_SOLVE_MAX_SPLIT, max_split)):
+            if v is None:
+                continue
+            st = _lib.lib().psp_hip_set_option(self._ctx(), opt, int(v))
+            if st != _lib.OK:
+                _raise(st, self._h, "set_option")
+
     # ---- Initialize ------------------------------------------------------
     def Initialize(self, velocity, panelScheme=CLIQUE_LDL_SELINV_1D):
         """Factor every panel (initialize_finalize.hpp:12-377)."""

@@ -172,3 +172,55 @@ def test_sweep_graph_replays_match_eager_bitwise(torch_cuda):
         torch.cuda.synchronize()
         assert torch.equal(y, eager), rep
         del junk
+
+
+WIDE = [(8, (12, 10, 24, 12)), (13, (16, 16, 20, 5)), (40, (9, 13, 27, 3)), (64, (12, 10, 24, 12))]
+
+
+@pytest.mark.parametrize("k,grid", WIDE)
+def test_wide_block_panel_solves_and_sweep_match_oracle(torch_cuda, k, grid):
+    """k >= 8 right-hand sides run the front solves on the matrix cores
+    (FP64 MFMA for FP64 fronts): panel solves and the sweep against the
+    oracle at 1e-10, including padded passes (k = 13, 40)."""
+    torch = torch_cuda
+    nx, ny, nz, cut = grid
+    disc, vel = _problem(nx, ny, nz)
+    o = _oracle(disc, vel, cut)
+    h = _helm(disc, vel, cutoff=cut)
+    rng = np.random.default_rng(30 + k)
+    l2n = o.local_to_natural()
+    for p in (0, o.num_panels // 2, o.num_panels - 1):
+        X = rand_block(rng, o.N, k)
+        xd = torch.from_numpy(X).cuda()
+        h.SolvePanel(xd, p)
+        torch.cuda.synchronize()
+        assert _rel(xd.cpu().numpy()[l2n], o.solve_panel(X[l2n], p)) < 1e-10, p
+    X = rand_block(rng, o.N, k)
+    xd = torch.from_numpy(X).cuda()
+    h.Precondition(xd)
+    torch.cuda.synchronize()
+    assert _rel(xd.cpu().numpy()[l2n], o.precondition(X[l2n])) < 1e-10
+
+
+@pytest.mark.parametrize("k", [8, 64])
+def test_wide_block_fp32_sweep_close_to_fp64(torch_cuda, k):
+    """FP32 fronts on FP32 MFMA (FP32 accumulation): the sweep agrees with the
+    FP64 sweep to FP32 accuracy, column by column."""
+    torch = torch_cuda
+    disc, vel = _problem(16, 14, 26)
+    h64 = _helm(disc, vel, "fp64")
+    h32 = _helm(disc, vel, "fp32")
+    rng = np.random.default_rng(40 + k)
+    X = torch.from_numpy(rand_block(rng, disc.nx * disc.ny * disc.nz, k)).cuda()
+    y64, y32 = X.clone(), X.clone()
+    h64.Precondition(y64)
+    h32.Precondition(y32)
+    torch.cuda.synchronize()
+    err = (torch.linalg.norm(y32 - y64, dim=0) / torch.linalg.norm(y64, dim=0)).max().item()
+    assert err < 1e-4, err
+    # narrow blocks (vector path) and wide blocks (matrix cores) agree
+    y8 = X[:, :4].contiguous()
+    h32.Precondition(y8)
+    torch.cuda.synchronize()
+    err2 = (torch.linalg.norm(y8 - y32[:, :4]) / torch.linalg.norm(y32[:, :4])).item()
+    assert err2 < 1e-5, err2
