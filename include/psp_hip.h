@@ -91,6 +91,7 @@ typedef struct {
     double bytes_factor;     /* bytes of stored factors                   */
     int num_panels;
     int num_fronts;          /* fronts over all panels                    */
+    double seconds_schur;    /* GPU time of the Schur-complement GEMMs    */
 } psp_hip_factor_report;
 
 typedef struct {

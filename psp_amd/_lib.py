@@ -55,6 +55,7 @@ class FactorReport(C.Structure):
         ("flops", C.c_double), ("schur_flops", C.c_double),
         ("bytes_factor", C.c_double),
         ("num_panels", C.c_int), ("num_fronts", C.c_int),
+        ("seconds_schur", C.c_double),
     ]
 
 

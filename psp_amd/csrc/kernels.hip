@@ -290,109 +290,6 @@ __global__ __launch_bounds__(kThreads) void k_ldl_trsm(const LdlItem* items, int
         if (c < m) F[r + (int64_t)(j0 + c) * ld] = x[c];
 }
 
-// --------------------------------------------------------------------------
-// Batched tile GEMM: C (+)= +/- opA(A) diag(D) opB(B) over a list of
-// 64x64 tiles (see GemmFlags).  Vector FMA path.
-// --------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_gemm(const GemmDesc* descs, const GemmTile* tiles,
-                                                    int tileStart, cx<T>* b0, int64_t st0,
-                                                    cx<T>* b1, int64_t st1, cx<T>* b2,
-                                                    int64_t st2, cx<T>* b3)
-{
-    constexpr int TM = 64, TK = 16;
-    const GemmTile tl = tiles[tileStart + blockIdx.x];
-    const GemmDesc g = descs[tl.desc];
-    const int bt = blockIdx.y;
-    cx<T>* bases[4] = {b0 + (int64_t)bt * st0, b1 + (int64_t)bt * st1, b2 + (int64_t)bt * st2,
-                       b3 + (int64_t)bt * st0};  // b3 (LT) has L's stride
-    const cx<T>* A = bases[g.abuf] + g.a;
-    const cx<T>* B = bases[g.bbuf] + g.b;
-    cx<T>* Cm = bases[g.cbuf] + g.c;
-    const cx<T>* D = bases[g.dbuf] + g.d;
-    const int tm = tl.tm, tn = tl.tn;
-    const int f = g.flags;
-    __shared__ cx<T> As[TK][TM + 1];
-    __shared__ cx<T> Bs[TK][TM + 1];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    cx<T> acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[a][c] = czero<T>();
-    int kbeg = 0, kend = g.K;
-    if (f & G_BUNIT) kbeg = tn;
-    if (f & G_AUNIT) kend = min(kend, tm + TM);
-    for (int k0 = kbeg; k0 < kend; k0 += TK) {
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const int e = tid + kThreads * l;
-            const int i = e & 63, kk = e >> 6;
-            const int gi = tm + i, gk = k0 + kk;
-            cx<T> v = czero<T>();
-            if (gi < g.M && gk < kend) {
-                if (f & G_AUNIT)
-                    v = gi > gk ? A[gi + (int64_t)gk * g.lda]
-                                : (gi == gk ? cx<T>{T(1), T(0)} : czero<T>());
-                else
-                    v = A[gi + (int64_t)gk * g.lda];
-                if (f & G_HASD) v = v * D[(int64_t)gk * g.dinc];
-            }
-            As[kk][i] = v;
-        }
-#pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            const int e = tid + kThreads * l;
-            int j, kk;
-            if (f & G_BTRANS) {
-                j = e & 63;
-                kk = e >> 6;
-            } else {
-                kk = e & 15;
-                j = e >> 4;
-            }
-            const int gj = tn + j, gk = k0 + kk;
-            cx<T> v = czero<T>();
-            if (gj < g.N && gk < kend) {
-                if (f & G_BTRANS)
-                    v = B[gj + (int64_t)gk * g.ldb];
-                else if (f & G_BUNIT)
-                    v = gk > gj ? B[gk + (int64_t)gj * g.ldb]
-                                : (gk == gj ? cx<T>{T(1), T(0)} : czero<T>());
-                else
-                    v = B[gk + (int64_t)gj * g.ldb];
-            }
-            Bs[kk][j] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < TK; ++kk) {
-            cx<T> a[4], bb[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                a[q] = As[kk][tx + 16 * q];
-                bb[q] = Bs[kk][ty + 16 * q];
-            }
-#pragma unroll
-            for (int p = 0; p < 4; ++p)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[p][q] = cfma(a[p], bb[q], acc[p][q]);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int i = tm + tx + 16 * p, j = tn + ty + 16 * q;
-            if (i >= g.M || j >= g.N) continue;
-            if ((f & G_CLOWER) && i < j) continue;
-            cx<T>* c = (f & G_CTRANS) ? Cm + j + (int64_t)i * g.ldc : Cm + i + (int64_t)j * g.ldc;
-            const cx<T> v = (f & G_NEG) ? -acc[p][q] : acc[p][q];
-            *c = (f & G_BETA0) ? v : (*c + v);
-        }
-}
-
 // Inverse of one unit-lower diagonal block in place (strict lower part).
 template <typename T, int NB>
 __global__ __launch_bounds__(kThreads) void k_inv_diag(const InvItem* items, cx<T>* L,
@@ -628,6 +525,24 @@ struct Mfma<double> {
 
 constexpr int kMmThreads = 256;  // 4 waves
 
+// Build-time tuning of the matrix-core solve kernel: factor loads in flight
+// per lane (FP32 / FP64 fronts) and the waves per SIMD the register
+// allocation must allow (0: the compiler's choice).
+#ifndef PSP_MVM_U32
+#define PSP_MVM_U32 16
+#endif
+#ifndef PSP_MVM_U64
+#define PSP_MVM_U64 8
+#endif
+#ifndef PSP_MVM_WAVES_PER_EU
+#define PSP_MVM_WAVES_PER_EU 0
+#endif
+#if PSP_MVM_WAVES_PER_EU > 0
+#define PSP_MVM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(PSP_MVM_WAVES_PER_EU)))
+#else
+#define PSP_MVM_OCCUPANCY
+#endif
+
 // Matrix-core path (k >= 8).  One workgroup per tile; wave w owns rows
 // 16 (w / WK) .. +16 of the tile and every WK-th 4-column step of its inner
 // range.  The inner range streams through LDS in chunks of CC columns (all
@@ -639,7 +554,7 @@ constexpr int kMmThreads = 256;  // 4 waves
 // after the tile record arrives, so a workgroup waits for few memory round
 // trips.
 template <typename T, int PH, int RT, int KB>
-__global__ __launch_bounds__(kMmThreads) void k_mvm(const SolveTile* tiles, int start,
+__global__ __launch_bounds__(kMmThreads) PSP_MVM_OCCUPANCY void k_mvm(const SolveTile* tiles, int start,
                                                      const int32_t* structs,
                                                      const int32_t* ginv, const cx<T>* L,
                                                      const cx<T>* LT, cx<T>* Xp, cx<T>* W,
@@ -647,7 +562,7 @@ __global__ __launch_bounds__(kMmThreads) void k_mvm(const SolveTile* tiles, int 
 {
     typedef typename Mfma<T>::acc acc_t;
     constexpr int NWV = kMmThreads / 64, WR = RT / 16, WK = NWV / WR, NB = KB / 8;
-    constexpr int CC = 2048 / KB, U = sizeof(T) == 4 ? 16 : 8;  // 8 KB of factor loads per wave
+    constexpr int CC = 2048 / KB, U = sizeof(T) == 4 ? PSP_MVM_U32 : PSP_MVM_U64;
     constexpr bool PRE = KB <= 16 && PH == PH_F;
     static_assert(WR * WK == NWV && CC % 4 == 0, "tile shape");
     static_assert(WK == 1 || 2 * CC * KB >= WR * NB * 256, "reduction scratch must fit in vs");
@@ -773,6 +688,224 @@ __global__ __launch_bounds__(kMmThreads) void k_mvm(const SolveTile* tiles, int 
             }
         }
     }
+}
+
+// --------------------------------------------------------------------------
+// Complex tile engine on the matrix cores: one 256-thread workgroup computes
+// an RT x 64 complex tile C = A B over an inner range, K staged through LDS
+// 16 columns at a time with two buffers (the global loads of stage s + 1
+// are in flight while stage s is multiplied).  Waves split the tile into
+// WRW x WCW parts; each wave holds RB x CB accumulator blocks of 16 rows x 8
+// complex columns (16 real MFMA columns: real parts, then imaginary parts).
+// A and B are read through policy objects: a(i, k) and b(k, j) return the
+// complex entry or zero outside the operand.
+// --------------------------------------------------------------------------
+template <int RT>
+struct TileShape {
+    static constexpr int WRW = RT == 64 ? 2 : 1;       // waves along rows
+    static constexpr int WCW = 4 / WRW;                // waves along columns
+    static constexpr int RB = RT / 16 / WRW;           // 16-row blocks per wave
+    static constexpr int CB = 8 / WCW;                 // 8-column blocks per wave
+};
+
+constexpr int kBK = 16;  // inner columns per LDS stage (solve tiles)
+
+template <typename T, int RT, bool BJFAST, int BK = kBK, class LA, class LB>
+__device__ __forceinline__ void MmaTile(const LA& la, const LB& lb, int kbeg, int kend,
+                                        cx<T> (*As)[BK][RT + 1], cx<T> (*Bs)[BK][65],
+                                        typename Mfma<T>::acc (&acc)[TileShape<RT>::RB]
+                                                                   [TileShape<RT>::CB])
+{
+    typedef TileShape<RT> S;
+    constexpr int kBK = BK;
+    constexpr int NA = RT * kBK / 256, NBL = 64 * kBK / 256;  // loads per thread per stage
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave / S::WCW, wc = wave % S::WCW;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < S::RB; ++a)
+#pragma unroll
+        for (int b = 0; b < S::CB; ++b) acc[a][b] = typename Mfma<T>::acc{0, 0, 0, 0};
+    if (kend <= kbeg) return;
+    cx<T> ra[NA], rb[NBL];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int e = tid + 256 * q, i = e % RT, kk = e / RT;
+            ra[q] = k0 + kk < kend ? la(i, k0 + kk) : czero<T>();
+        }
+#pragma unroll
+        for (int q = 0; q < NBL; ++q) {
+            const int e = tid + 256 * q;
+            const int j = BJFAST ? e % 64 : e / kBK, kk = BJFAST ? e / 64 : e % kBK;
+            rb[q] = k0 + kk < kend ? lb(k0 + kk, j) : czero<T>();
+        }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int e = tid + 256 * q;
+            As[buf][e / RT][e % RT] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < NBL; ++q) {
+            const int e = tid + 256 * q;
+            const int j = BJFAST ? e % 64 : e / kBK, kk = BJFAST ? e / 64 : e % kBK;
+            Bs[buf][kk][j] = rb[q];
+        }
+    };
+    const int nst = (kend - kbeg + kBK - 1) / kBK;
+    fetch(kbeg);
+    __syncthreads();  // the caller's previous use of As / Bs is over
+    put(0);
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) fetch(kbeg + (st + 1) * kBK);
+#pragma unroll
+        for (int k4 = 0; k4 < kBK / 4; ++k4) {
+            cx<T> av[S::RB];
+#pragma unroll
+            for (int a = 0; a < S::RB; ++a) av[a] = As[buf][4 * k4 + lk][16 * (wr * S::RB + a) + li];
+#pragma unroll
+            for (int b = 0; b < S::CB; ++b) {
+                const cx<T> v = Bs[buf][4 * k4 + lk][8 * (wc * S::CB + b) + (li & 7)];
+                const T b1 = li < 8 ? v.r : v.i;
+                const T b2 = li < 8 ? -v.i : v.r;
+#pragma unroll
+                for (int a = 0; a < S::RB; ++a) {
+                    acc[a][b] = Mfma<T>::step(av[a].r, b1, acc[a][b]);
+                    acc[a][b] = Mfma<T>::step(av[a].i, b2, acc[a][b]);
+                }
+            }
+        }
+        if (st + 1 < nst) put(buf ^ 1);
+        __syncthreads();
+    }
+}
+
+// Calls f(i, j, value) for every tile element this lane finishes (lanes with
+// l % 16 < 8 after pairing the real and imaginary halves).
+template <typename T, int RT, class F>
+__device__ __forceinline__ void MmaForEach(const typename Mfma<T>::acc (&acc)[TileShape<RT>::RB]
+                                                                            [TileShape<RT>::CB],
+                                           const F& f)
+{
+    typedef TileShape<RT> S;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave / S::WCW, wc = wave % S::WCW;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < S::RB; ++a)
+#pragma unroll
+        for (int b = 0; b < S::CB; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const T val = acc[a][b][q];
+                const T other = __shfl_xor(val, 8, 64);
+                if (li < 8)
+                    f(16 * (wr * S::RB + a) + Mfma<T>::row(lk, q), 8 * (wc * S::CB + b) + li,
+                      cx<T>{val, other});
+            }
+}
+
+// Wide-block front solves (k >= 32): one workgroup per solve tile and pass
+// of 64 right-hand sides, through the tile engine.
+template <typename T, int PH, int RT>
+__global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
+                                              const int32_t* structs, const int32_t* ginv,
+                                              const cx<T>* L, const cx<T>* LT, cx<T>* Xp,
+                                              cx<T>* W, cx<T>* P, int k)
+{
+    typedef TileShape<RT> S;
+    __shared__ cx<T> As[2][kBK][RT + 1];
+    __shared__ cx<T> Bs[2][kBK][65];
+    const SolveTile tl = tiles[start + blockIdx.x];
+    const NodeDev nd = tl.nd;
+    const int s = nd.s, fs = nd.s + nd.u;
+    const cx<T>* Lf = L + nd.l;
+    const cx<T>* Lt = LT + nd.l;
+    for (int r0 = 0; r0 < k; r0 += 64) {
+        const int kc = min(64, k - r0);
+        auto la = [&](int i, int col) {
+            return i < tl.nrows ? FactorEntry<T, PH>(Lf, Lt, s, fs, tl.row0 + i, col)
+                                : czero<T>();
+        };
+        auto lb = [&](int col, int j) {
+            return j < kc ? InnerOperand<T, PH>(nd, structs, ginv, Xp, W, col, r0 + j, k)
+                          : czero<T>();
+        };
+        typename Mfma<T>::acc acc[S::RB][S::CB];
+        MmaTile<T, RT, true>(la, lb, tl.col0, tl.col1, As, Bs, acc);
+        MmaForEach<T, RT>(acc, [&](int i, int j, cx<T> a) {
+            if (i >= tl.nrows || j >= kc) return;
+            if (tl.slot >= 0)
+                P[((int64_t)tl.slot * 64 + i) * k + r0 + j] = a;
+            else
+                FinishGather<T, PH>(nd, ginv, Xp, W, tl.row0 + i, r0 + j, k, a);
+        });
+    }
+}
+
+// Batched tile GEMM on the matrix cores: C (+)= +/- opA(A) diag(D) opB(B)
+// over a list of 64x64 tiles (see GemmFlags), through the tile engine.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_mma(const GemmDesc* descs, const GemmTile* tiles,
+                                                  int tileStart, cx<T>* b0, int64_t st0,
+                                                  cx<T>* b1, int64_t st1, cx<T>* b2,
+                                                  int64_t st2, cx<T>* b3)
+{
+    constexpr int BK = kBK;  // inner columns per stage (32 measured slower on the factor)
+    __shared__ cx<T> As[2][BK][65];
+    __shared__ cx<T> Bs[2][BK][65];
+    const GemmTile tl = tiles[tileStart + blockIdx.x];
+    const GemmDesc g = descs[tl.desc];
+    const int bt = blockIdx.y;
+    cx<T>* bases[4] = {b0 + (int64_t)bt * st0, b1 + (int64_t)bt * st1, b2 + (int64_t)bt * st2,
+                       b3 + (int64_t)bt * st0};  // b3 (LT) has L's stride
+    const cx<T>* A = bases[g.abuf] + g.a;
+    const cx<T>* B = bases[g.bbuf] + g.b;
+    cx<T>* Cm = bases[g.cbuf] + g.c;
+    const cx<T>* D = bases[g.dbuf] + g.d;
+    const int tm = tl.tm, tn = tl.tn, f = g.flags;
+    int kbeg = 0, kend = g.K;
+    if (f & G_BUNIT) kbeg = tn;
+    if (f & G_AUNIT) kend = min(kend, tm + 64);
+    auto la = [&](int i, int k) {
+        const int gi = tm + i;
+        if (gi >= g.M) return czero<T>();
+        cx<T> v;
+        if (f & G_AUNIT)
+            v = gi > k ? A[gi + (int64_t)k * g.lda] : (gi == k ? cx<T>{T(1), T(0)} : czero<T>());
+        else
+            v = A[gi + (int64_t)k * g.lda];
+        if (f & G_HASD) v = v * D[(int64_t)k * g.dinc];
+        return v;
+    };
+    auto lbt = [&](int k, int j) {
+        const int gj = tn + j;
+        return gj < g.N ? B[gj + (int64_t)k * g.ldb] : czero<T>();
+    };
+    auto lbn = [&](int k, int j) {
+        const int gj = tn + j;
+        if (gj >= g.N) return czero<T>();
+        if (f & G_BUNIT)
+            return k > gj ? B[k + (int64_t)gj * g.ldb] : (k == gj ? cx<T>{T(1), T(0)} : czero<T>());
+        return B[k + (int64_t)gj * g.ldb];
+    };
+    typename Mfma<T>::acc acc[TileShape<64>::RB][TileShape<64>::CB];
+    if (f & G_BTRANS)
+        MmaTile<T, 64, true, BK>(la, lbt, kbeg, kend, As, Bs, acc);
+    else
+        MmaTile<T, 64, false, BK>(la, lbn, kbeg, kend, As, Bs, acc);
+    MmaForEach<T, 64>(acc, [&](int i, int j, cx<T> v) {
+        const int gi = tm + i, gj = tn + j;
+        if (gi >= g.M || gj >= g.N) return;
+        if ((f & G_CLOWER) && gi < gj) return;
+        cx<T>* c = (f & G_CTRANS) ? Cm + gj + (int64_t)gi * g.ldc : Cm + gi + (int64_t)gj * g.ldc;
+        if (f & G_NEG) v = -v;
+        *c = (f & G_BETA0) ? v : (*c + v);
+    });
 }
 
 // Sum of the inner slices of split row blocks, in slot order, then the
@@ -1185,7 +1318,7 @@ void LaunchGemm(const ClassDev& c, int tileStart, int tileCount, hipStream_t s)
 {
     if (tileCount <= 0) return;
     cx<T>* T0 = (cx<T>*)c.T;
-    hipLaunchKernelGGL((k_gemm<T>), dim3(tileCount, c.npanels), dim3(kThreads), 0, s, c.gemms,
+    hipLaunchKernelGGL((k_gemm_mma<T>), dim3(tileCount, c.npanels), dim3(256), 0, s, c.gemms,
                        c.tiles, tileStart, (cx<T>*)c.L, c.fstore, (cx<T>*)c.U, c.ustore, T0,
                        c.tstore, (cx<T>*)c.LT);
 }
@@ -1212,7 +1345,17 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
         if (r.count <= 0) return;
         constexpr int PH = decltype(ph)::value;
         if (prof) (void)hipEventRecord(prof->a, s);
-        if (k >= 8) {
+        if (k >= 32) {
+            if (r.rows == 64)
+                hipLaunchKernelGGL((k_mmw<T, PH, 64>), dim3(r.count), dim3(256), 0, s, c.mv,
+                                   r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+            else if (r.rows == 32)
+                hipLaunchKernelGGL((k_mmw<T, PH, 32>), dim3(r.count), dim3(256), 0, s, c.mv,
+                                   r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+            else
+                hipLaunchKernelGGL((k_mmw<T, PH, 16>), dim3(r.count), dim3(256), 0, s, c.mv,
+                                   r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+        } else if (k >= 8) {
             auto launch = [&](auto rt, auto kb) {
                 constexpr int RT = decltype(rt)::value, KB = decltype(kb)::value;
                 hipLaunchKernelGGL((k_mvm<T, PH, RT, KB>), dim3(r.count), dim3(kMmThreads), 0, s,
@@ -1228,12 +1371,8 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
             };
             if (k <= 8)
                 byRows(std::integral_constant<int, 8>());
-            else if (k <= 16)
-                byRows(std::integral_constant<int, 16>());
-            else if (k <= 32)
-                byRows(std::integral_constant<int, 32>());
             else
-                byRows(std::integral_constant<int, 64>());
+                byRows(std::integral_constant<int, 16>());
         } else if (r.rows == 64)
             hipLaunchKernelGGL((k_mv<T, PH, 64>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
                                r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);

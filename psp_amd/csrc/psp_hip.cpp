@@ -400,13 +400,15 @@ int RebuildSolveTiles(psp_hip_ctx* c)
 }
 
 template <typename T>
-int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv)
+int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv, double* tSchur)
 {
     const StencilArgs st = Stencil(c);
     hipStream_t s = c->stream;
     hipEvent_t e[4];
     for (auto& ev : e) HIPCHK(c, hipEventCreate(&ev));
     float acc[3] = {0, 0, 0};
+    double schur = 0;
+    std::vector<hipEvent_t> se;  // around every level's Schur-complement GEMM
     for (int cl = 0; cl < 3; ++cl) {
         if (!c->plan.present[cl]) continue;
         const ClassPlan& P = c->plan.cls[cl];
@@ -426,7 +428,14 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv)
                 LaunchLdlTrsm<T>(D, S.ldlTrsm[kb].first, S.ldlTrsm[kb].second, nb, s);
                 LaunchGemm<T>(D, S.trail[kb].tileStart, S.trail[kb].tileCount, s);
             }
-            LaunchGemm<T>(D, S.schur.tileStart, S.schur.tileCount, s);
+            if (S.schur.tileCount > 0) {
+                se.resize(se.size() + 2);
+                HIPCHK(c, hipEventCreate(&se[se.size() - 2]));
+                HIPCHK(c, hipEventCreate(&se[se.size() - 1]));
+                HIPCHK(c, hipEventRecord(se[se.size() - 2], s));
+                LaunchGemm<T>(D, S.schur.tileStart, S.schur.tileCount, s);
+                HIPCHK(c, hipEventRecord(se[se.size() - 1], s));
+            }
         }
         HIPCHK(c, hipEventRecord(e[2], s));
         LaunchInvDiag<T>(D, nb, s);
@@ -447,7 +456,14 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv)
         HIPCHK(c, hipEventElapsedTime(&ms, e[2], e[3]));
         acc[2] += ms;
     }
+    for (size_t i = 0; i + 1 < se.size(); i += 2) {
+        float ms;
+        HIPCHK(c, hipEventElapsedTime(&ms, se[i], se[i + 1]));
+        schur += ms * 1e-3;
+    }
+    for (auto& ev : se) (void)hipEventDestroy(ev);
     for (auto& ev : e) (void)hipEventDestroy(ev);
+    *tSchur = schur;
     *tFill = acc[0] * 1e-3;
     *tLdl = acc[1] * 1e-3;
     *tInv = acc[2] * 1e-3;
@@ -849,8 +865,9 @@ int psp_hip_factor(psp_hip_ctx* c, const double* velocity, int scheme, psp_hip_f
     HIPCHK(c, hipMemcpyAsync(c->vel, velocity, sizeof(double) * N, hipMemcpyHostToDevice,
                              c->stream));
     double tf = 0, tl = 0, ti = 0;
-    st = c->desc.precision == PSP_HIP_FRONTS_FP32 ? FactorClasses<float>(c, &tf, &tl, &ti)
-                                                  : FactorClasses<double>(c, &tf, &tl, &ti);
+    double tsc = 0;
+    st = c->desc.precision == PSP_HIP_FRONTS_FP32 ? FactorClasses<float>(c, &tf, &tl, &ti, &tsc)
+                                                  : FactorClasses<double>(c, &tf, &tl, &ti, &tsc);
     if (st) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->factored = true;
@@ -859,6 +876,7 @@ int psp_hip_factor(psp_hip_ctx* c, const double* velocity, int scheme, psp_hip_f
         rep->seconds_fill = tf;
         rep->seconds_ldl = tl;
         rep->seconds_inverse = ti;
+        rep->seconds_schur = tsc;
         double fl = 0, sc = 0, by = 0;
         int nf = 0;
         for (int cl = 0; cl < 3; ++cl) {
