@@ -91,6 +91,131 @@ def cpu_baseline(wl, out, args):
                        f"cycle; extrapolated to {its} iterations and {np_} panels")}
 
 
+def cpu_full(wl, args):
+    """The oracle's whole Solve (factor + GMRES) on the host cores, timed."""
+    from oracle.oracle import Oracle, num_threads
+    d, vel, B = wl.disc, wl.velocity, wl.B
+    o = Oracle(d.omega, d.nx, d.ny, d.nz, d.wx, d.wy, d.wz, sigma=d.sigmax)
+    t0 = time.perf_counter()
+    o.factor(vel)
+    t_fac = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    x, st, rep = o.solve(B, args.m, args.tol)
+    t_sol = time.perf_counter() - t0
+    log(f"[cpu] factor {t_fac:.2f}s, solve {t_sol:.2f}s, {rep.iterations} its")
+    return {"value": B.shape[1] / t_sol, "unit": "RHS/s", "cores": num_threads(), "kind": "port",
+            "factor_s": t_fac, "solve_s": t_sol,
+            "sample": "oracle (FP64 CPU restatement, not the PSP binary): the whole workload"}
+
+
+def bench_batch(args, dev):
+    """32 uneven problems (n in 40..96, 4..16 sources each) solved together:
+    aggregate RHS/s = total right-hand sides / wall time of one SolveBatch."""
+    import torch
+    import psp_amd
+    from psp_amd import workloads
+    wls = workloads.config4()
+    ktot = sum(w.B.shape[1] for w in wls)
+    res = {}
+    for prec in ("fp32", "fp64"):
+        if prec == "fp64" and args.no_fp64:
+            continue
+        # FP32 fronts: all 32 problems resident and solved together.  FP64
+        # fronts take twice the HBM, so they go in 4 groups of 8.
+        group = len(wls) if prec == "fp32" else 8
+        t_fac, t_sol, reps = 0.0, 0.0, []
+        for g0 in range(0, len(wls), group):
+            part = wls[g0:g0 + group]
+            helms = []
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for w in part:
+                h = psp_amd.DistHelmholtz(w.disc, precision=prec)
+                h.Initialize(w.velocity)
+                helms.append(h)
+            torch.cuda.synchronize()
+            t_fac += time.perf_counter() - t0
+            srcs = [torch.from_numpy(w.B).to(dev) for w in part]
+            for w_ in range(args.warmup):
+                psp_amd.SolveBatch(helms, [b.clone() for b in srcs], args.m, args.tol)
+            times = []
+            for s_ in range(args.steps):
+                bufs = [b.clone() for b in srcs]
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                rp = psp_amd.SolveBatch(helms, bufs, args.m, args.tol)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t0)
+                del bufs
+            t_sol += sum(times) / len(times)
+            reps += rp
+            del helms, srcs
+            torch.cuda.empty_cache()
+        res[prec] = dict(factor_all_s=t_fac, solve_s=t_sol, rhs_per_s=ktot / t_sol,
+                         problems_per_batch=group,
+                         max_final_rel=max(r.final_max_rel for r in reps),
+                         iterations=[r.iterations for r in reps])
+        log(f"[bench] batch32 {prec} ({group} problems per SolveBatch): factor all {t_fac:.2f}s, "
+            f"solve all {t_sol:.3f}s -> {ktot / t_sol:.1f} RHS/s, worst final rel "
+            f"{res[prec]['max_final_rel']:.2e}")
+    line = {"metric": "aggregate RHS solves/s over 32 uneven problems to 1e-5 rel. residual",
+            "value": res["fp32"]["rhs_per_s"], "unit": "RHS/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * res["fp32"]["solve_s"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "c64 fronts / c128 GMRES",
+            "data": "synthetic",
+            "config": {"workload": "32 uneven bump problems n in {40..96}, 4-16 rhs each",
+                       "rhs_total": ktot, "sizes": [w.disc.nx for w in wls],
+                       "rhs": [w.B.shape[1] for w in wls]},
+            "paths": res}
+    print(json.dumps(line), flush=True)
+
+
+def bench_adjoint(args, dev):
+    """80^3 misfit gradient with 16 sources: forward solve, adjoint solve with
+    the same factors, gradient kernel; gradients/s, FP32 vs FP64 fronts."""
+    import torch
+    from psp_amd import workloads
+    from psp_amd.adjoint import HelmholtzMisfit
+    import psp_amd
+    wl = workloads.config5()
+    F = torch.from_numpy(wl.B).to(dev)
+    rec = torch.from_numpy(wl.receivers).to(dev)
+    ht = psp_amd.DistHelmholtz(wl.disc)
+    ht.Initialize(1.0 / np.sqrt(wl.m_true))
+    Ut = F.clone()
+    ht.Solve(Ut, args.m, 1e-10)
+    data = Ut[rec].clone()
+    del ht, Ut
+    res, grads = {}, {}
+    for prec in ("fp32", "fp64"):
+        opts = {"m": args.m, "tol": args.tol, "precision": prec}
+        times = []
+        for s_ in range(args.warmup + args.steps):
+            m = torch.from_numpy(wl.m).to(dev).requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss = HelmholtzMisfit.apply(m, F, rec, data, wl.disc, opts)
+            loss.backward()
+            torch.cuda.synchronize()
+            if s_ >= args.warmup:
+                times.append(time.perf_counter() - t0)
+        grads[prec] = m.grad.detach()
+        t = sum(times) / len(times)
+        res[prec] = dict(seconds=t, gradients_per_s=1.0 / t, loss=loss.item())
+        log(f"[bench] adjoint80 {prec}: {t:.3f}s per gradient (factor + 2 solves of 16 rhs)")
+    err = (torch.linalg.norm(grads["fp32"] - grads["fp64"]) / torch.linalg.norm(grads["fp64"])).item()
+    line = {"metric": "misfit gradients/s (factor + forward + adjoint solve, 16 sources)",
+            "value": res["fp32"]["gradients_per_s"], "unit": "gradients/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * res["fp32"]["seconds"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "c64 fronts / c128 GMRES",
+            "data": "synthetic",
+            "config": {"workload": wl.name, "rel_tol": args.tol},
+            "paths": res, "fp32_vs_fp64_grad_rel_err": err}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +227,12 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-front leg")
+    ap.add_argument("--workload", default="bumps100",
+                    choices=["bumps100", "unitcube40", "thrustbelt", "batch32", "adjoint80"],
+                    help="bumps100 (default, the headline line): 100^3 bumps, --k sources; "
+                         "unitcube40: 40^3 c=1, 1 source; thrustbelt: 200x200x100, 64 sources; "
+                         "batch32: 32 uneven problems solved together; adjoint80: 80^3 "
+                         "misfit gradient, 16 sources")
     ap.add_argument("--calibrate", action="store_true",
                     help="copy a 1 GiB device buffer once (a known-size kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -121,8 +252,17 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
 
-    log(f"[bench] building workload n={args.n} k={args.k}")
-    wl = workloads.config2(n=args.n, k=args.k)
+    if args.workload == "batch32":
+        return bench_batch(args, dev)
+    if args.workload == "adjoint80":
+        return bench_adjoint(args, dev)
+    if args.workload == "unitcube40":
+        wl = workloads.config1()
+    elif args.workload == "thrustbelt":
+        wl = workloads.config3()
+    else:
+        log(f"[bench] building workload n={args.n} k={args.k}")
+        wl = workloads.config2(n=args.n, k=args.k)
     disc, vel, B = wl.disc, wl.velocity, wl.B
     k = B.shape[1]
     Bsrc = torch.from_numpy(B).to(dev)
@@ -161,20 +301,14 @@ def main():
         for s, r in enumerate(reps):
             log(f"[bench] {precision} step {s}: {r.iterations} its, rel {r.final_max_rel:.2e}, "
                 f"{r.seconds_total:.3f}s")
-        return h, rep, t_fac, t, reps, bufs[0]
+        x0 = bufs[0]
+        del bufs
+        return h, rep, t_fac, t, reps, x0
 
     h32, f32, tf32, t32, reps32, x32 = run_path("fp32", args.steps, args.warmup)
     value = args.steps * k / t32
     out["fp32"] = dict(factor_s=tf32, solve_s=t32 / args.steps, iterations=reps32[0].iterations,
                        rhs_per_s=value, final_rel=reps32[0].final_max_rel)
-
-    if not args.no_fp64:
-        h64, f64, tf64, t64, reps64, x64 = run_path("fp64", 1, 0)
-        err = (torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64)).item()
-        out["fp64"] = dict(factor_s=tf64, solve_s=t64, iterations=reps64[0].iterations,
-                           rhs_per_s=k / t64, final_rel=reps64[0].final_max_rel)
-        out["fp32_vs_fp64_rel_err"] = err
-        log(f"[bench] fp32 vs fp64 solution rel err {err:.3e}")
 
     # ---- roofline of the dominant kernels: the front-solve products -----
     # One more sweep of the same block, launched kernel by kernel with HIP
@@ -182,16 +316,41 @@ def main():
     # the sweep from a hipGraph, whose kernels cannot be bracketed singly).
     launches, ksec, kbytes, kflops = h32.ProfileSweep(Bsrc.clone())
     achieved = kbytes / ksec / 1e9
+    kname = "k_mvm" if k >= 8 and k < 32 else ("k_mmw" if k >= 32 else "k_mv")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                "frac": achieved / 8000.0, "traffic": measured_traffic(),
-                "kernel": "k_mv (front-solve products, c64 fronts)",
+                "frac": achieved / 8000.0, "traffic": measured_traffic() if k == 8 else None,
+                "kernel": f"{kname} (front-solve products, c64 fronts)",
                 "launches_per_sweep": launches, "avg_launch_us": 1e6 * ksec / launches,
                 "algorithmic_bytes_per_launch": kbytes / launches,
-                "achieved_tflops": kflops / ksec / 1e12}
-    log(f"[bench] roofline: {launches} k_mv launches/sweep, avg {1e6 * ksec / launches:.1f} us, "
-        f"{kbytes / launches / 1e6:.2f} MB/launch -> {achieved:.0f} GB/s")
+                "achieved_tflops": kflops / ksec / 1e12,
+                "schur_gemm_tflops": f32.schur_flops / max(f32.seconds_schur, 1e-12) / 1e12}
+    log(f"[bench] roofline: {launches} front-solve launches/sweep, avg "
+        f"{1e6 * ksec / launches:.1f} us, {kbytes / launches / 1e6:.2f} MB/launch -> "
+        f"{achieved:.0f} GB/s, {kflops / ksec / 1e12:.2f} TFLOP/s; Schur GEMMs "
+        f"{roofline['schur_gemm_tflops']:.1f} TFLOP/s")
+    del h32
+    torch.cuda.empty_cache()
 
-    cpu = None if args.no_cpu else cpu_baseline(wl, out, args)
+    if not args.no_fp64:
+        h64, f64, tf64, t64, reps64, x64 = run_path("fp64", 1, 0)
+        err = (torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64)).item()
+        out["fp64"] = dict(factor_s=tf64, solve_s=t64, iterations=reps64[0].iterations,
+                           rhs_per_s=k / t64, final_rel=reps64[0].final_max_rel,
+                           schur_gemm_tflops=f64.schur_flops / max(f64.seconds_schur, 1e-12) / 1e12)
+        out["fp32_vs_fp64_rel_err"] = err
+        out["fp32_over_fp64_rhs_per_s"] = value / (k / t64)
+        log(f"[bench] fp32 vs fp64 solution rel err {err:.3e}")
+        del h64
+        torch.cuda.empty_cache()
+
+    if args.no_cpu:
+        cpu = None
+    elif args.workload == "unitcube40":
+        cpu = cpu_full(wl, args)
+    elif args.workload == "bumps100":
+        cpu = cpu_baseline(wl, out, args)
+    else:
+        cpu = None  # the CPU restatement needs hours here; see the 100^3 line
 
     line = {
         "metric": "RHS solves/s to 1e-5 rel. residual (FP32 fronts + FP64 GMRES)",

@@ -165,6 +165,16 @@ int psp_hip_solve_panel(psp_hip_ctx* ctx, void* B, int k, int panel);
 int psp_hip_profile_sweep(psp_hip_ctx* ctx, void* B, int k,
                           psp_hip_kernel_stats* stats);
 
+/* Many independent solves at once (no reference counterpart; for callers
+ * with batches of uneven problems): solve i runs psp_hip_solve(ctxs[i],
+ * Bs[i], ks[i], m, relTol, maxIts, flags, &reports[i]) on its own host
+ * thread and its context's stream, so the sweeps (each replayed from its
+ * own hipGraph) of different problems share the GPU.  Contexts must be
+ * distinct.  reports may be NULL.  Returns the first failing status, or OK. */
+int psp_hip_solve_batch(psp_hip_ctx* const* ctxs, void* const* Bs, const int* ks, int n,
+                        int m, double relTol, int maxIts, int flags,
+                        psp_hip_solve_report* reports);
+
 /* GMRES with a fixed number of inner steps, for parity at a fixed
  * iteration count; relHist (host, steps doubles) gets max_r rel. residual. */
 int psp_hip_solve_fixed(psp_hip_ctx* ctx, void* B, int k, int m, int steps,

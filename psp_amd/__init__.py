@@ -7,10 +7,10 @@ this package is a thin binding over it.
 from .helmholtz import (CLIQUE_LDL_1D, CLIQUE_LDL_SELINV_1D, DIRICHLET, PML,
                         BreakdownError, Discretization, DistHelmholtz,
                         DistUniformGrid, GridData, LogicError, RuntimeError_,
-                        unit_cube_discretization)
+                        SolveBatch, unit_cube_discretization)
 
 __all__ = [
     "CLIQUE_LDL_1D", "CLIQUE_LDL_SELINV_1D", "DIRICHLET", "PML", "BreakdownError",
     "Discretization", "DistHelmholtz", "DistUniformGrid", "GridData", "LogicError",
-    "RuntimeError_", "unit_cube_discretization",
+    "RuntimeError_", "SolveBatch", "unit_cube_discretization",
 ]

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <algorithm>
 #include <vector>
 
@@ -49,6 +50,7 @@ struct psp_hip_ctx {
     // solve workspace
     int wsK = 0, wsM = 0;
     std::vector<void*> wsAllocs;
+    std::vector<void*> facAllocs;  // Schur-complement and inversion scratch, only while factoring
     void *Xp = nullptr, *W = nullptr;
     void* P = nullptr;  // partial sums of split solve row blocks (64 rows x k per slot)
     c128 *V = nullptr, *x = nullptr, *x0 = nullptr, *w = nullptr, *b = nullptr;
@@ -296,8 +298,6 @@ int EnsureDevice(psp_hip_ctx* c)
         std::vector<c128> s3c = ToC128(s3a);
         if ((st = Upload(c, &D.s3art, s3c))) return st;
         if ((st = Alloc(c, &D.L, eb * (size_t)D.npanels * D.fstore, c->allocs))) return st;
-        if ((st = Alloc(c, &D.U, eb * (size_t)D.npanels * D.ustore, c->allocs))) return st;
-        if ((st = Alloc(c, &D.T, eb * (size_t)D.npanels * D.tstore, c->allocs))) return st;
         if ((st = Alloc(c, &D.LT, eb * (size_t)D.npanels * D.fstore, c->allocs))) return st;
     }
     c->devReady = true;
@@ -311,6 +311,16 @@ void FreeGraphs(psp_hip_ctx* c)
         if (g.graph) (void)hipGraphDestroy(g.graph);
     }
     c->graphs.clear();
+}
+
+void FreeFactorScratch(psp_hip_ctx* c)
+{
+    for (void* p : c->facAllocs) (void)hipFree(p);
+    c->facAllocs.clear();
+    for (int cl = 0; cl < 3; ++cl) {
+        c->dev[cl].U = nullptr;
+        c->dev[cl].T = nullptr;
+    }
 }
 
 void FreeWorkspace(psp_hip_ctx* c)
@@ -803,6 +813,7 @@ int psp_hip_destroy(psp_hip_ctx* c)
     if (!c) return PSP_HIP_OK;
     if (c->devReady) (void)hipStreamSynchronize(c->stream);
     FreeWorkspace(c);
+    FreeFactorScratch(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
@@ -864,12 +875,29 @@ int psp_hip_factor(psp_hip_ctx* c, const double* velocity, int scheme, psp_hip_f
     const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
     HIPCHK(c, hipMemcpyAsync(c->vel, velocity, sizeof(double) * N, hipMemcpyHostToDevice,
                              c->stream));
+    // The Schur-complement (U) and inversion (T) scratch lives only while
+    // factoring, so that many factored problems fit in HBM at once.
+    const size_t eb = FrontBytes(c);
+    for (int cl = 0; cl < 3; ++cl) {
+        if (!c->plan.present[cl]) continue;
+        ClassDev& D = c->dev[cl];
+        if ((st = Alloc(c, &D.U, eb * (size_t)D.npanels * D.ustore, c->facAllocs)) ||
+            (st = Alloc(c, &D.T, eb * (size_t)D.npanels * D.tstore, c->facAllocs))) {
+            FreeFactorScratch(c);
+            return st;
+        }
+    }
     double tf = 0, tl = 0, ti = 0;
     double tsc = 0;
     st = c->desc.precision == PSP_HIP_FRONTS_FP32 ? FactorClasses<float>(c, &tf, &tl, &ti, &tsc)
                                                   : FactorClasses<double>(c, &tf, &tl, &ti, &tsc);
-    if (st) return st;
+    if (st) {
+        (void)hipStreamSynchronize(c->stream);
+        FreeFactorScratch(c);
+        return st;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    FreeFactorScratch(c);
     c->factored = true;
     if (rep) {
         rep->seconds_total = Now() - t0;
@@ -900,6 +928,31 @@ int psp_hip_solve(psp_hip_ctx* c, void* B, int k, int m, double relTol, int maxI
                   psp_hip_solve_report* rep)
 {
     return RunSolve(c, B, k, m, relTol, maxIts, flags, 0, nullptr, rep);
+}
+
+int psp_hip_solve_batch(psp_hip_ctx* const* ctxs, void* const* Bs, const int* ks, int n,
+                        int m, double relTol, int maxIts, int flags,
+                        psp_hip_solve_report* reports)
+{
+    if (n < 0 || (n > 0 && (!ctxs || !Bs || !ks))) return PSP_HIP_ERR_INVALID;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return PSP_HIP_ERR_INVALID;
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i])
+                return Fail(ctxs[i], PSP_HIP_ERR_INVALID, "solve_batch: contexts must be distinct");
+    }
+    std::vector<int> st(n, PSP_HIP_OK);
+    std::vector<std::thread> workers;
+    workers.reserve(n);
+    for (int i = 0; i < n; ++i)
+        workers.emplace_back([&, i]() {
+            st[i] = RunSolve(ctxs[i], Bs[i], ks[i], m, relTol, maxIts, flags, 0, nullptr,
+                             reports ? &reports[i] : nullptr);
+        });
+    for (std::thread& t : workers) t.join();
+    for (int i = 0; i < n; ++i)
+        if (st[i] != PSP_HIP_OK) return st[i];
+    return PSP_HIP_OK;
 }
 
 int psp_hip_solve_fixed(psp_hip_ctx* c, void* B, int k, int m, int steps, int flags,

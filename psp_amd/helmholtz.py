@@ -336,6 +336,49 @@ class DistHelmholtz:
         return _lib.lib().psp_hip_reordered_index(self._ctx(), x, y, vl, vs)
 
 
+def SolveBatch(helms, Bs, m=20, relTol=1e-4, maxIts=0, raise_on_error=True):
+    """Solves many independent problems at once: problem i is
+    helms[i].Solve(Bs[i], m, relTol, maxIts), each on its own host thread and
+    stream, so that uneven problems fill the GPU together.  The blocks are
+    all numpy arrays (host) or all torch complex128 CUDA tensors (HBM).
+    Returns the SolveReports in order."""
+    n = len(helms)
+    if len(Bs) != n:
+        raise LogicError("SolveBatch: one right-hand-side block per problem")
+    if len({id(h) for h in helms}) != n:
+        raise LogicError("SolveBatch: problems must be distinct DistHelmholtz objects")
+    dev = [_is_torch_cuda(b) for b in Bs]
+    if n and any(d != dev[0] for d in dev):
+        raise LogicError("SolveBatch: blocks must be all host or all device")
+    ctxs = (C.c_void_p * n)()
+    ptrs = (C.c_void_p * n)()
+    ks = (C.c_int * n)()
+    for i, (h, b) in enumerate(zip(helms, Bs)):
+        ctxs[i] = h._ctx()
+        _lib.lib().psp_hip_set_stream(ctxs[i], None)  # each problem on its own stream
+        if dev[i]:
+            import torch
+            assert b.dtype == torch.complex128 and b.is_contiguous()
+            ptrs[i] = b.data_ptr()
+            ks[i] = b.shape[1] if b.dim() == 2 else 1
+        else:
+            if not (isinstance(b, np.ndarray) and b.dtype == np.complex128
+                    and b.flags.c_contiguous):
+                raise LogicError("B must be a C-contiguous complex128 array")
+            ptrs[i] = b.ctypes.data
+            ks[i] = b.shape[1] if b.ndim == 2 else 1
+    if n and dev[0]:
+        import torch
+        torch.cuda.current_stream().synchronize()  # the blocks are ready
+    reps = (SolveReport * n)()
+    st = _lib.lib().psp_hip_solve_batch(ctxs, ptrs, ks, n, m, relTol, maxIts,
+                                        _lib.DEVICE_POINTER if (n and dev[0]) else 0, reps)
+    if st != _lib.OK and raise_on_error:
+        bad = next((i for i in range(n) if reps[i].status != _lib.OK), 0)
+        _raise(st, helms[bad]._h, "SolveBatch")
+    return list(reps)
+
+
 def unit_cube_discretization(n, nz=None, ppw=10.0, cmin=1.0, pml=5, sigma=1.5, ny=None):
     """Helper: grid spacing h = 1/(n+1) and omega from PSP's points-per-
     wavelength definition (initialize_finalize.hpp:60-69):

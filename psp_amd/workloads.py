@@ -215,3 +215,21 @@ def config5(n=80, k=16, pml=5):
     receivers = np.arange(zr * n * n, (zr + 1) * n * n, dtype=np.int64)
     return InversionWorkload(f"adjoint bumps {n}^3 10ppw {k}src", disc, 1.0 / vel ** 2,
                              1.0 / vel_true ** 2, B, receivers)
+
+
+def config4(count=32):
+    """32 independent problems: n_i uniform in {40, 48, ..., 96} (cubes),
+    bump velocities in [1.5, 4.5] (the 100^3 recipe, seeded per problem),
+    10 PPW at each problem's c_min, k_i uniform in {4, ..., 16} Gaussian
+    sources at seeded interior points."""
+    rng = _rng(4, 3)
+    sizes = rng.choice(np.arange(40, 97, 8), size=count)
+    ks = rng.integers(4, 17, size=count)
+    out = []
+    for i, (n, k) in enumerate(zip(sizes, ks)):
+        n, k = int(n), int(k)
+        vel = bump_velocity(n, n, n, cfg=4000 + i)
+        disc = discretization(n, n, n, float(vel.min()), 10.0)
+        B = gaussian_sources(n, n, n, interior_points(n, n, n, k, cfg=4000 + i))
+        out.append(Workload(f"batch[{i}] bumps {n}^3 10ppw {k}rhs", disc, vel, B))
+    return out

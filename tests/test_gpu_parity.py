@@ -224,3 +224,31 @@ def test_wide_block_fp32_sweep_close_to_fp64(torch_cuda, k):
     torch.cuda.synchronize()
     err2 = (torch.linalg.norm(y8 - y32[:, :4]) / torch.linalg.norm(y32[:, :4])).item()
     assert err2 < 1e-5, err2
+
+
+def test_solve_batch_matches_individual_solves(torch_cuda):
+    """Uneven problems solved together (one host thread and stream each)
+    give exactly the results and iteration counts of one-by-one solves."""
+    torch = torch_cuda
+    import psp_amd
+    from psp_amd import workloads
+    probs = [(14, 12, 24, 3, "fp64"), (16, 16, 20, 9, "fp32"), (12, 10, 26, 5, "fp64"),
+             (18, 14, 22, 33, "fp32")]
+    helms, blocks, ref, its = [], [], [], []
+    for j, (nx, ny, nz, k, prec) in enumerate(probs):
+        disc, vel = _problem(nx, ny, nz, cfg_seed=50 + j)
+        h = _helm(disc, vel, prec)
+        B = workloads.gaussian_sources(nx, ny, nz, workloads.interior_points(nx, ny, nz, k, cfg=60 + j))
+        x = torch.from_numpy(B).cuda()
+        r = h.Solve(x, 20, 1e-9)
+        torch.cuda.synchronize()
+        helms.append(h)
+        blocks.append(torch.from_numpy(B).cuda())
+        ref.append(x.cpu())
+        its.append(r.iterations)
+    reps = psp_amd.SolveBatch(helms, blocks, 20, 1e-9)
+    torch.cuda.synchronize()
+    for j in range(len(probs)):
+        assert reps[j].status == 0 and reps[j].final_max_rel < 1e-9
+        assert reps[j].iterations == its[j], j
+        assert torch.equal(blocks[j].cpu(), ref[j]), j
