@@ -165,6 +165,15 @@ int psp_hip_solve_panel(psp_hip_ctx* ctx, void* B, int k, int panel);
 int psp_hip_profile_sweep(psp_hip_ctx* ctx, void* B, int k,
                           psp_hip_kernel_stats* stats);
 
+/* Tile-shape tuner for the front solves (no reference counterpart): for
+ * each panel class, tree level and direction, times every candidate shape
+ * (rows per tile 64/32/16, inner slices 1/2/4/8) with k right-hand sides on
+ * the class's first panel and keeps the fastest; the chosen shapes stay for
+ * later solves of this context.  before/after (may be NULL) receive the
+ * estimated front-solve time of one sweep with the old and new shapes.
+ * Needs a factored context. */
+int psp_hip_autotune(psp_hip_ctx* ctx, int k, double* before, double* after);
+
 /* Many independent solves at once (no reference counterpart; for callers
  * with batches of uneven problems): solve i runs psp_hip_solve(ctxs[i],
  * Bs[i], ks[i], m, relTol, maxIts, flags, &reports[i]) on its own host

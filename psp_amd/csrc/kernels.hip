@@ -1334,7 +1334,7 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s)
 
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, void* Pv, int k,
-                      hipStream_t s, MvProfile* prof)
+                      hipStream_t s, MvProfile* prof, int onlyLevel, int onlyPhase)
 {
     const cx<T>* L = (const cx<T>*)c.L + (int64_t)b * c.fstore;
     const cx<T>* LT = (const cx<T>*)c.LT + (int64_t)b * c.fstore;
@@ -1397,6 +1397,14 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
             hipLaunchKernelGGL((k_mv_reduce<T, PH>), dim3(r.rcount), dim3(kThreads), 0, s,
                                c.mvred, r.rstart, c.ginv, Xp, W, P, k);
     };
+    if (onlyLevel >= 0) {  // one (level, phase) launch, for tile-shape tuning
+        const ClassPlan::SolveLevel& lv = plan.solveLevels[onlyLevel];
+        if (onlyPhase == PH_F)
+            mv(std::integral_constant<int, PH_F>(), lv, lv.f);
+        else
+            mv(std::integral_constant<int, PH_B>(), lv, lv.b);
+        return;
+    }
     for (int d = plan.maxDepth; d >= 0; --d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
         mv(std::integral_constant<int, PH_F>(), lv, lv.f);
@@ -1528,7 +1536,7 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
     template void LaunchGemm<T>(const ClassDev&, int, int, hipStream_t);                          \
     template void LaunchInvDiag<T>(const ClassDev&, int, hipStream_t);                            \
     template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, void*, int,  \
-                                      hipStream_t, MvProfile*);                                   \
+                                      hipStream_t, MvProfile*, int, int);                         \
     template void LaunchTransposeScale<T>(const ClassDev&, hipStream_t);                          \
     template void LaunchFoldBack<T>(const ClassDev&, hipStream_t);                                \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \

@@ -108,7 +108,8 @@ struct MvProfile {
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xp, void* W,
                       void* P,
-                      int k, hipStream_t s, MvProfile* prof = nullptr);
+                      int k, hipStream_t s, MvProfile* prof = nullptr, int onlyLevel = -1,
+                      int onlyPhase = -1);
 template <typename T>
 void LaunchTransposeScale(const ClassDev& c, hipStream_t s);
 template <typename T>

@@ -50,6 +50,7 @@ struct psp_hip_ctx {
     // solve workspace
     int wsK = 0, wsM = 0;
     std::vector<void*> wsAllocs;
+    int64_t wsSlots = 0;  // partial-sum slots the workspace holds
     std::vector<void*> facAllocs;  // Schur-complement and inversion scratch, only while factoring
     void *Xp = nullptr, *W = nullptr;
     void* P = nullptr;  // partial sums of split solve row blocks (64 rows x k per slot)
@@ -331,9 +332,17 @@ void FreeWorkspace(psp_hip_ctx* c)
     c->wsK = c->wsM = 0;
 }
 
+int64_t NeededSlots(const psp_hip_ctx* c)
+{
+    int64_t maxs = 1;
+    for (int cl = 0; cl < 3; ++cl)
+        if (c->plan.present[cl]) maxs = std::max<int64_t>(maxs, c->plan.cls[cl].maxSlots);
+    return maxs;
+}
+
 int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
 {
-    if (c->wsK == k && c->wsM >= m) return PSP_HIP_OK;
+    if (c->wsK == k && c->wsM >= m && c->wsSlots >= NeededSlots(c)) return PSP_HIP_OK;
     FreeWorkspace(c);
     const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
     const size_t eb = FrontBytes(c);
@@ -372,6 +381,7 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
     c->ydev = (c128*)p;
     c->wsK = k;
     c->wsM = m;
+    c->wsSlots = maxs;
     return PSP_HIP_OK;
 }
 
@@ -784,6 +794,88 @@ int RunSolve(psp_hip_ctx* c, void* B, int k, int m, double relTol, int maxIts, i
 
 }  // namespace
 
+namespace {
+
+// Tile-shape tuner for the front solves: for every panel class, level and
+// phase, times the launch (products + reduce) of each candidate shape on the
+// class's first panel and keeps the fastest.  The estimate of one sweep's
+// product time weighs each class by its panel solves per sweep (2P - 1 in
+// all: every panel forward, every panel but the last backward).
+template <typename T>
+int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
+{
+    int rc = EnsureWorkspace(c, k, 1);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    hipEvent_t ea, eb;
+    HIPCHK(c, hipEventCreate(&ea));
+    HIPCHK(c, hipEventCreate(&eb));
+    const Geometry& g = c->plan.geo;
+    double tb = 0, ta = 0;
+    for (int cl = 0; cl < 3 && !rc; ++cl) {
+        if (!c->plan.present[cl]) continue;
+        ClassPlan& P = c->plan.cls[cl];
+        ClassDev& D = c->dev[cl];
+        double weight = 0;
+        for (int p = 0; p < g.numPanels; ++p)
+            if (g.cls(p) == cl) weight += p + 1 < g.numPanels ? 2 : 1;
+        auto timeShape = [&](int d, int ph, SolveShape sh, double* t) -> int {
+            P.shapes[d][ph] = sh;
+            BuildSolveTiles(P);
+            int st;
+            if ((st = ReUpload(c, &D.mv, SolveTiles(P, P.mv))) ||
+                (st = ReUpload(c, &D.mvred, SolveTiles(P, P.mvred))) ||
+                (st = EnsureWorkspace(c, k, 1)))
+                return st;
+            const size_t eb_ = FrontBytes(c);
+            LaunchFillZero(c->Xp, (int64_t)(eb_ * P.n * k), s);
+            LaunchFillZero(c->W, (int64_t)(eb_ * P.wstore * k), s);
+            LaunchPanelSolve<T>(D, P, 0, c->Xp, c->W, c->P, k, s, nullptr, d, ph);  // warm
+            constexpr int kReps = 3;
+            HIPCHK(c, hipEventRecord(ea, s));
+            for (int r = 0; r < kReps; ++r)
+                LaunchPanelSolve<T>(D, P, 0, c->Xp, c->W, c->P, k, s, nullptr, d, ph);
+            HIPCHK(c, hipEventRecord(eb, s));
+            HIPCHK(c, hipEventSynchronize(eb));
+            HIPCHK(c, hipGetLastError());
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, ea, eb));
+            *t = ms * 1e-3 / kReps;
+            return PSP_HIP_OK;
+        };
+        for (int d = 0; d <= P.maxDepth && !rc; ++d)
+            for (int ph = 0; ph < 2 && !rc; ++ph) {
+                const SolveShape cur = P.shapes[d][ph];
+                SolveShape best = cur;
+                double tcur = 0;
+                if ((rc = timeShape(d, ph, cur, &tcur))) break;
+                double tbest = tcur;
+                for (int rows : {64, 32, 16})
+                    for (int split : {1, 2, 4, 8}) {
+                        if (rc || (rows == cur.rows && split == cur.split)) continue;
+                        double t = 0;
+                        if ((rc = timeShape(d, ph, SolveShape{rows, split}, &t))) break;
+                        if (t < 0.97 * tbest) {
+                            tbest = t;
+                            best = SolveShape{rows, split};
+                        }
+                    }
+                P.shapes[d][ph] = best;
+                tb += weight * tcur;
+                ta += weight * tbest;
+            }
+    }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    if (rc) return rc;
+    if ((rc = RebuildSolveTiles(c))) return rc;
+    if (before) *before = tb;
+    if (after) *after = ta;
+    return PSP_HIP_OK;
+}
+
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -981,6 +1073,15 @@ int psp_hip_precondition(psp_hip_ctx* c, void* B, int k)
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PSP_HIP_OK;
+}
+
+int psp_hip_autotune(psp_hip_ctx* c, int k, double* before, double* after)
+{
+    if (!c) return PSP_HIP_ERR_INVALID;
+    if (!c->factored) return Fail(c, PSP_HIP_ERR_STATE, "autotune needs Initialize first");
+    if (k < 1 || k > 256) return Fail(c, PSP_HIP_ERR_INVALID, "k out of range [1, 256]");
+    return c->desc.precision == PSP_HIP_FRONTS_FP32 ? Autotune<float>(c, k, before, after)
+                                                    : Autotune<double>(c, k, before, after);
 }
 
 int psp_hip_profile_sweep(psp_hip_ctx* c, void* B, int k, psp_hip_kernel_stats* stats)

@@ -192,6 +192,16 @@ class DistHelmholtz:
         if st != _lib.OK:
             _raise(st, self._h, "set_option")
 
+    def Autotune(self, k):
+        """Picks the front-solve tile shapes by timing (see psp_hip_autotune);
+        returns the estimated front-solve seconds of one sweep before and
+        after."""
+        b, a = C.c_double(), C.c_double()
+        st = _lib.lib().psp_hip_autotune(self._ctx(), int(k), C.byref(b), C.byref(a))
+        if st != _lib.OK:
+            _raise(st, self._h, "Autotune")
+        return b.value, a.value
+
     def set_solve_tiling(self, min_tiles=None, max_split=None):
         """Default front-solve tile shapes: workgroups wanted per launch and
         the most inner slices per row block (rebuilds the solve tiles)."""

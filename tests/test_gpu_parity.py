@@ -252,3 +252,22 @@ def test_solve_batch_matches_individual_solves(torch_cuda):
         assert reps[j].status == 0 and reps[j].final_max_rel < 1e-9
         assert reps[j].iterations == its[j], j
         assert torch.equal(blocks[j].cpu(), ref[j]), j
+
+
+def test_autotuned_shapes_keep_parity(torch_cuda):
+    """Tile shapes picked by the autotuner change only the summation order:
+    the sweep still matches the oracle."""
+    torch = torch_cuda
+    disc, vel = _problem(16, 14, 26)
+    o = _oracle(disc, vel)
+    h = _helm(disc, vel)
+    before, after = h.Autotune(8)
+    assert 0 < after <= before
+    rng = np.random.default_rng(71)
+    l2n = o.local_to_natural()
+    for k in (3, 8, 40):
+        X = rand_block(rng, o.N, k)
+        xd = torch.from_numpy(X).cuda()
+        h.Precondition(xd)
+        torch.cuda.synchronize()
+        assert _rel(xd.cpu().numpy()[l2n], o.precondition(X[l2n])) < 1e-10, k
