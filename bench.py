@@ -33,7 +33,7 @@ def measured_traffic():
     """HBM bytes per k_mv launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc)
     recorded in the committed profile summary, or None."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_mv_front_solves_bumps100_8rhs.md")
+    path = os.path.join(ROOT, "profiles", "k_mvm_front_solves_bumps100_8rhs.md")
     try:
         m = re.search(r"\*\*([0-9.]+) MB per launch\*\* of HBM traffic", open(path).read())
         return float(m.group(1)) * 1e6 if m else None
@@ -225,8 +225,14 @@ def main():
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--m", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--max-its", type=int, default=-1,
+                    help="GMRES iteration cap (0: none, as the reference; default: none, "
+                         "100 for thrustbelt)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-front leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default front-solve tile shapes (no tuning pass after "
+                         "Initialize)")
     ap.add_argument("--workload", default="bumps100",
                     choices=["bumps100", "unitcube40", "thrustbelt", "batch32", "adjoint80"],
                     help="bumps100 (default, the headline line): 100^3 bumps, --k sources; "
@@ -256,6 +262,8 @@ def main():
         return bench_batch(args, dev)
     if args.workload == "adjoint80":
         return bench_adjoint(args, dev)
+    if args.max_its < 0:
+        args.max_its = 100 if args.workload == "thrustbelt" else 0
     if args.workload == "unitcube40":
         wl = workloads.config1()
     elif args.workload == "thrustbelt":
@@ -284,10 +292,18 @@ def main():
         log(f"[bench] {precision}: factor {t_fac:.3f}s (fill {rep.seconds_fill:.3f} "
             f"ldl {rep.seconds_ldl:.3f} inv {rep.seconds_inverse:.3f}), "
             f"{rep.flops / 1e12:.2f} TFLOP, {rep.bytes_factor / 1e9:.2f} GB of factors")
+        tuned = None
+        if not args.no_autotune:
+            t0 = time.perf_counter()
+            before, after = h.Autotune(k)
+            tuned = dict(seconds=time.perf_counter() - t0, est_products_before_s=before,
+                         est_products_after_s=after)
+            log(f"[bench] {precision}: autotuned solve tiles in {tuned['seconds']:.2f}s "
+                f"(est. front-solve time per sweep {before * 1e3:.2f} -> {after * 1e3:.2f} ms)")
         bufs = [Bsrc.clone() for _ in range(max(steps, 1))]
         for w in range(warmup):
             x = Bsrc.clone()
-            r = h.Solve(x, args.m, args.tol)
+            r = h.Solve(x, args.m, args.tol, args.max_its, raise_on_error=False)
             log(f"[bench] {precision} warmup {w}: {r.iterations} its, rel {r.final_max_rel:.2e}, "
                 f"{r.seconds_total:.3f}s (precond {r.seconds_precond:.3f}, "
                 f"multiply {r.seconds_multiply:.3f}, arnoldi {r.seconds_arnoldi:.3f})")
@@ -295,20 +311,22 @@ def main():
         t0 = time.perf_counter()
         reps = []
         for s in range(steps):
-            reps.append(h.Solve(bufs[s], args.m, args.tol))
+            reps.append(h.Solve(bufs[s], args.m, args.tol, args.max_its, raise_on_error=False))
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         for s, r in enumerate(reps):
             log(f"[bench] {precision} step {s}: {r.iterations} its, rel {r.final_max_rel:.2e}, "
-                f"{r.seconds_total:.3f}s")
+                f"{r.seconds_total:.3f}s" + ("" if r.status == 0 else f" (status {r.status})"))
         x0 = bufs[0]
         del bufs
+        rep.tuned = tuned
         return h, rep, t_fac, t, reps, x0
 
     h32, f32, tf32, t32, reps32, x32 = run_path("fp32", args.steps, args.warmup)
     value = args.steps * k / t32
     out["fp32"] = dict(factor_s=tf32, solve_s=t32 / args.steps, iterations=reps32[0].iterations,
-                       rhs_per_s=value, final_rel=reps32[0].final_max_rel)
+                       converged=reps32[0].status == 0,
+                       rhs_per_s=value, final_rel=reps32[0].final_max_rel, autotune=f32.tuned)
 
     # ---- roofline of the dominant kernels: the front-solve products -----
     # One more sweep of the same block, launched kernel by kernel with HIP
@@ -318,7 +336,8 @@ def main():
     achieved = kbytes / ksec / 1e9
     kname = "k_mvm" if k >= 8 and k < 32 else ("k_mmw" if k >= 32 else "k_mv")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                "frac": achieved / 8000.0, "traffic": measured_traffic() if k == 8 else None,
+                "frac": achieved / 8000.0,
+                "traffic": measured_traffic() if (k == 8 and args.workload == "bumps100") else None,
                 "kernel": f"{kname} (front-solve products, c64 fronts)",
                 "launches_per_sweep": launches, "avg_launch_us": 1e6 * ksec / launches,
                 "algorithmic_bytes_per_launch": kbytes / launches,
@@ -335,7 +354,9 @@ def main():
         h64, f64, tf64, t64, reps64, x64 = run_path("fp64", 1, 0)
         err = (torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64)).item()
         out["fp64"] = dict(factor_s=tf64, solve_s=t64, iterations=reps64[0].iterations,
+                           converged=reps64[0].status == 0,
                            rhs_per_s=k / t64, final_rel=reps64[0].final_max_rel,
+                           autotune=f64.tuned,
                            schur_gemm_tflops=f64.schur_flops / max(f64.seconds_schur, 1e-12) / 1e12)
         out["fp32_vs_fp64_rel_err"] = err
         out["fp32_over_fp64_rhs_per_s"] = value / (k / t64)
@@ -366,7 +387,8 @@ def main():
         "dtype": "c64 fronts / c128 GMRES",
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": [disc.nx, disc.ny, disc.nz], "rhs": k,
-                   "gmres_m": args.m, "rel_tol": args.tol, "omega": disc.omega},
+                   "gmres_m": args.m, "rel_tol": args.tol, "omega": disc.omega,
+                   "max_its": args.max_its},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "paths": out,

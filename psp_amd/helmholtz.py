@@ -162,7 +162,10 @@ class DistHelmholtz:
 
     # ---- lifecycle -------------------------------------------------------
     def __del__(self):
-        self.Finalize()
+        try:
+            self.Finalize()
+        except Exception:  # interpreter shutdown: the library may be gone already
+            pass
 
     def Finalize(self):
         h = getattr(self, "_h", None)
