@@ -818,13 +818,33 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
                                               cx<T>* W, cx<T>* P, int k)
 {
     typedef TileShape<RT> S;
+    constexpr int IX = 2048;  // inner indices whose gather addresses are cached in LDS
     __shared__ cx<T> As[2][kBK][RT + 1];
     __shared__ cx<T> Bs[2][kBK][65];
+    __shared__ int32_t ix[PH == PH_F ? 2 * IX : IX];
     const SolveTile tl = tiles[start + blockIdx.x];
     const NodeDev nd = tl.nd;
     const int s = nd.s, fs = nd.s + nd.u;
     const cx<T>* Lf = L + nd.l;
     const cx<T>* Lt = LT + nd.l;
+    // The gather addresses of the inner range (F: the two children's update
+    // rows feeding each front row; B: the panel rows of the update columns)
+    // are read once into LDS, so each K stage waits for one memory round
+    // trip instead of two.
+    const int ncols = tl.col1 - tl.col0;
+    const bool cached = ncols <= IX;
+    if (cached) {
+        for (int c = threadIdx.x; c < ncols; c += 256) {
+            const int col = tl.col0 + c;
+            if (PH == PH_F) {
+                ix[2 * c] = ginv[2 * (nd.w + col)];
+                ix[2 * c + 1] = ginv[2 * (nd.w + col) + 1];
+            } else {
+                ix[c] = col >= s ? structs[nd.ustart + col - s] : -1;
+            }
+        }
+    }
+    __syncthreads();  // the cache is read by every thread's stage loads
     for (int r0 = 0; r0 < k; r0 += 64) {
         const int kc = min(64, k - r0);
         auto la = [&](int i, int col) {
@@ -832,8 +852,17 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
                                 : czero<T>();
         };
         auto lb = [&](int col, int j) {
-            return j < kc ? InnerOperand<T, PH>(nd, structs, ginv, Xp, W, col, r0 + j, k)
-                          : czero<T>();
+            if (j >= kc) return czero<T>();
+            if (!cached) return InnerOperand<T, PH>(nd, structs, ginv, Xp, W, col, r0 + j, k);
+            const int c = col - tl.col0, rr = r0 + j;
+            if (PH == PH_F) {
+                cx<T> v = col < s ? Xp[(int64_t)(nd.off + col) * k + rr] : czero<T>();
+                const int g0 = ix[2 * c], g1 = ix[2 * c + 1];
+                if (g0 >= 0) v = v + W[(int64_t)g0 * k + rr];
+                if (g1 >= 0) v = v + W[(int64_t)g1 * k + rr];
+                return v;
+            }
+            return col < s ? W[(nd.w + col) * k + rr] : Xp[(int64_t)ix[c] * k + rr];
         };
         typename Mfma<T>::acc acc[S::RB][S::CB];
         MmaTile<T, RT, true>(la, lb, tl.col0, tl.col1, As, Bs, acc);
