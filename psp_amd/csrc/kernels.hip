@@ -325,36 +325,48 @@ __global__ __launch_bounds__(kThreads) void k_inv_diag(const InvItem* items, cx<
 // Front solves on the padded panel vector Xp (rows x k, interleaved), with
 // the factors in the solve layout (rows < s of L: Linv with a stored unit
 // diagonal; rows >= s: -L_BL Linv; LT: [Linv^T D^-1 | (-L_BL Linv)^T]).
-// Per node, with g = [Xp_own; 0] + the children's update vectors (gathered
-// through ginv), W the node's work rows (s + u):
-//   forward  F: W_top = Linv g_top,  W_bot = g_bot + (-L_BL Linv) g_top;
+// Per node, with g = [Xp_own; 0] + the children's update vectors (read from
+// per-slot copies of the node's front rows that the children write):
+//   forward  F: W_top = Linv g_top; the update g_bot + (-L_BL Linv) g_top
+//               goes into the parent's slot copy;
 //   backward B: Xp_own = LT [W_top; x_struct].
 // Each is one product over the node's factor block, split into row tiles
 // (and, near the root, inner slices summed by a reduce launch).
 // --------------------------------------------------------------------------
 enum { PH_F = 0, PH_B = 1 };
 
-// Forward input of front row r: the panel right-hand side on the node's own
-// rows plus the children's update vectors (extend-add through ginv).
+// The children's update rows: each child writes its rows straight into a
+// per-slot copy of its parent's front rows (slot 0: first child, slot 1:
+// second child), at the parent rows given by relinds.  Rows a child does not
+// touch stay zero from the workspace's allocation (each slot copy belongs to
+// one panel class, and a class writes the same rows in every solve), so a
+// parent reads its gathered input with plain loads.
 template <typename T>
-__device__ __forceinline__ cx<T> Gathered(const NodeDev& nd, const int32_t* ginv, const cx<T>* Xp,
-                                          const cx<T>* W, int r, int rr, int k)
+struct Slots {
+    cx<T>* G;                // slot 0; slot 1 at G + half
+    int64_t half;            // work rows of the class * k
+    const int32_t* relinds;  // row in the parent's front of each update row
+};
+
+// Forward input of front row r: the panel right-hand side on the node's own
+// rows plus the children's update vectors.
+template <typename T>
+__device__ __forceinline__ cx<T> Gathered(const NodeDev& nd, const Slots<T>& gs, const cx<T>* Xp,
+                                          int r, int rr, int k)
 {
-    cx<T> v = r < nd.s ? Xp[(int64_t)(nd.off + r) * k + rr] : czero<T>();
-    const int64_t wr = nd.w + r;
-    const int g0 = ginv[2 * wr], g1 = ginv[2 * wr + 1];
-    if (g0 >= 0) v = v + W[(int64_t)g0 * k + rr];
-    if (g1 >= 0) v = v + W[(int64_t)g1 * k + rr];
+    const int64_t g = (nd.w + r) * k + rr;
+    cx<T> v = gs.G[g] + gs.G[gs.half + g];
+    if (r < nd.s) v = v + Xp[(int64_t)(nd.off + r) * k + rr];
     return v;
 }
 
 // Inner operand of a phase at inner index col, right-hand side rr.
 template <typename T, int PH>
 __device__ __forceinline__ cx<T> InnerOperand(const NodeDev& nd, const int32_t* structs,
-                                              const int32_t* ginv, const cx<T>* Xp,
+                                              const Slots<T>& gs, const cx<T>* Xp,
                                               const cx<T>* W, int col, int rr, int k)
 {
-    if (PH == PH_F) return Gathered(nd, ginv, Xp, W, col, rr, k);
+    if (PH == PH_F) return Gathered(nd, gs, Xp, col, rr, k);
     if (col < nd.s) return W[(nd.w + col) * k + rr];
     return Xp[(int64_t)structs[nd.ustart + col - nd.s] * k + rr];
 }
@@ -368,26 +380,35 @@ __device__ __forceinline__ cx<T> FactorEntry(const cx<T>* Lf, const cx<T>* Lt, i
     return col >= row ? Lt[row + (int64_t)col * s] : czero<T>();
 }
 
-// Finish of output row `row`, right-hand side r, product a: F writes W
-// (top rows: y; bottom rows: gathered + a), B writes the node's Xp rows.
-// For F bottom rows `g` is the gathered value.
+// Finish of output row `row`, right-hand side r, product a.  F: top rows
+// are y (into W); a bottom row is g + a, with g the gathered value of that
+// row, written into the parent's slot copy at parent row `prow`.  B: the
+// node's Xp rows.
 template <typename T, int PH>
-__device__ __forceinline__ void Finish(const NodeDev& nd, cx<T>* Xp, cx<T>* W, int row, int r,
-                                       int k, cx<T> a, cx<T> g)
+__device__ __forceinline__ void Finish(const NodeDev& nd, const Slots<T>& gs, cx<T>* Xp, cx<T>* W,
+                                       int row, int r, int k, cx<T> a, cx<T> g, int prow)
 {
-    if (PH == PH_F)
-        W[(nd.w + row) * k + r] = row < nd.s ? a : g + a;
-    else
+    if (PH == PH_F) {
+        if (row < nd.s)
+            W[(nd.w + row) * k + r] = a;
+        else if (nd.cslot >= 0)
+            gs.G[nd.cslot * gs.half + (int64_t)(nd.pw + prow) * k + r] = g + a;
+    } else {
         Xp[(int64_t)(nd.off + row) * k + r] = a;
+    }
 }
 
 template <typename T, int PH>
-__device__ __forceinline__ void FinishGather(const NodeDev& nd, const int32_t* ginv, cx<T>* Xp,
+__device__ __forceinline__ void FinishGather(const NodeDev& nd, const Slots<T>& gs, cx<T>* Xp,
                                              cx<T>* W, int row, int r, int k, cx<T> a)
 {
-    const cx<T> g = (PH == PH_F && row >= nd.s) ? Gathered(nd, ginv, Xp, W, row, r, k)
-                                                : czero<T>();
-    Finish<T, PH>(nd, Xp, W, row, r, k, a, g);
+    if (PH == PH_F && row >= nd.s) {
+        const int64_t g = (nd.w + row) * k + r;
+        Finish<T, PH>(nd, gs, Xp, W, row, r, k, a, gs.G[g] + gs.G[gs.half + g],
+                      gs.relinds[nd.ustart + row - nd.s]);
+    } else {
+        Finish<T, PH>(nd, gs, Xp, W, row, r, k, a, czero<T>(), 0);
+    }
 }
 
 constexpr int kMvThreads = 512;  // 8 waves split each row block's inner indices
@@ -404,7 +425,7 @@ constexpr int kMvThreads = 512;  // 8 waves split each row block's inner indices
 template <typename T, int PH, int RT>
 __global__ __launch_bounds__(kMvThreads) void k_mv(const SolveTile* tiles, int start,
                                                     const int32_t* structs,
-                                                    const int32_t* ginv, const cx<T>* L,
+                                                    Slots<T> gs, const cx<T>* L,
                                                     const cx<T>* LT, cx<T>* Xp, cx<T>* W,
                                                     cx<T>* P, int k)
 {
@@ -439,7 +460,7 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const SolveTile* tiles, int s
             __syncthreads();  // the previous chunk's readers are done with vs
             for (int e = tid; e < ncol * KC; e += kMvThreads) {
                 const int c = e / KC, r = e % KC;
-                vs[c][r] = r < kc ? InnerOperand<T, PH>(nd, structs, ginv, Xp, W, cb + c, r0 + r, k)
+                vs[c][r] = r < kc ? InnerOperand<T, PH>(nd, structs, gs, Xp, W, cb + c, r0 + r, k)
                                   : czero<T>();
             }
             __syncthreads();
@@ -480,7 +501,7 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const SolveTile* tiles, int s
                     if (tl.slot >= 0)
                         P[((int64_t)tl.slot * 64 + rl) * k + r0 + r] = acc[r];
                     else
-                        FinishGather<T, PH>(nd, ginv, Xp, W, row, r0 + r, k, acc[r]);
+                        FinishGather<T, PH>(nd, gs, Xp, W, row, r0 + r, k, acc[r]);
                 }
         }
     }
@@ -556,7 +577,7 @@ constexpr int kMmThreads = 256;  // 4 waves
 template <typename T, int PH, int RT, int KB>
 __global__ __launch_bounds__(kMmThreads) PSP_MVM_OCCUPANCY void k_mvm(const SolveTile* tiles, int start,
                                                      const int32_t* structs,
-                                                     const int32_t* ginv, const cx<T>* L,
+                                                     Slots<T> gs, const cx<T>* L,
                                                      const cx<T>* LT, cx<T>* Xp, cx<T>* W,
                                                      cx<T>* P, int k)
 {
@@ -582,15 +603,24 @@ __global__ __launch_bounds__(kMmThreads) PSP_MVM_OCCUPANCY void k_mvm(const Solv
     // 16 wr + row(lk, q) of the tile and right-hand side 8 b + li of the pass.
     const bool fin = wk == 0 && li < 8 && tl.slot < 0;
     cx<T> pre[PRE ? NB : 1][4];
+    int prow[4] = {0, 0, 0, 0};
     if (PRE && fin && k <= KB) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = tl.row0 + 16 * wr + Mfma<T>::row(lk, q);
+            if (row < tl.row0 + tl.nrows && row >= s) prow[q] = gs.relinds[nd.ustart + row - s];
+        }
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = tl.row0 + 16 * wr + Mfma<T>::row(lk, q), rr = 8 * b + li;
-                pre[PRE ? b : 0][q] = (row < tl.row0 + tl.nrows && row >= s && rr < k)
-                                          ? Gathered(nd, ginv, Xp, W, row, rr, k)
-                                          : czero<T>();
+                cx<T> g = czero<T>();
+                if (row < tl.row0 + tl.nrows && row >= s && rr < k) {
+                    const int64_t o = (nd.w + row) * k + rr;
+                    g = gs.G[o] + gs.G[gs.half + o];
+                }
+                pre[PRE ? b : 0][q] = g;
             }
     }
     for (int r0 = 0; r0 < k; r0 += KB) {
@@ -618,7 +648,7 @@ __global__ __launch_bounds__(kMmThreads) PSP_MVM_OCCUPANCY void k_mvm(const Solv
             for (int e = tid; e < nst * KB; e += kMmThreads) {
                 const int c = e / KB, r = e % KB;
                 vs[c][r] = (r < kc && c < ncol)
-                               ? InnerOperand<T, PH>(nd, structs, ginv, Xp, W, cb + c, r0 + r, k)
+                               ? InnerOperand<T, PH>(nd, structs, gs, Xp, W, cb + c, r0 + r, k)
                                : czero<T>();
             }
             __syncthreads();
@@ -680,9 +710,10 @@ __global__ __launch_bounds__(kMmThreads) PSP_MVM_OCCUPANCY void k_mvm(const Solv
                         if (tl.slot >= 0)
                             P[((int64_t)tl.slot * 64 + rl) * k + r0 + rr] = a;
                         else if (PRE && k <= KB)
-                            Finish<T, PH>(nd, Xp, W, row, rr, k, a, pre[PRE ? b : 0][q]);
+                            Finish<T, PH>(nd, gs, Xp, W, row, rr, k, a, pre[PRE ? b : 0][q],
+                                          prow[q]);
                         else
-                            FinishGather<T, PH>(nd, ginv, Xp, W, row, r0 + rr, k, a);
+                            FinishGather<T, PH>(nd, gs, Xp, W, row, r0 + rr, k, a);
                     }
                 }
             }
@@ -813,7 +844,7 @@ __device__ __forceinline__ void MmaForEach(const typename Mfma<T>::acc (&acc)[Ti
 // of 64 right-hand sides, through the tile engine.
 template <typename T, int PH, int RT>
 __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
-                                              const int32_t* structs, const int32_t* ginv,
+                                              const int32_t* structs, Slots<T> gs,
                                               const cx<T>* L, const cx<T>* LT, cx<T>* Xp,
                                               cx<T>* W, cx<T>* P, int k)
 {
@@ -821,27 +852,21 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
     constexpr int IX = 2048;  // inner indices whose gather addresses are cached in LDS
     __shared__ cx<T> As[2][kBK][RT + 1];
     __shared__ cx<T> Bs[2][kBK][65];
-    __shared__ int32_t ix[PH == PH_F ? 2 * IX : IX];
+    __shared__ int32_t ix[PH == PH_F ? 1 : IX];
     const SolveTile tl = tiles[start + blockIdx.x];
     const NodeDev nd = tl.nd;
     const int s = nd.s, fs = nd.s + nd.u;
     const cx<T>* Lf = L + nd.l;
     const cx<T>* Lt = LT + nd.l;
-    // The gather addresses of the inner range (F: the two children's update
-    // rows feeding each front row; B: the panel rows of the update columns)
-    // are read once into LDS, so each K stage waits for one memory round
-    // trip instead of two.
+    // B: the panel rows of the update columns are read once into LDS, so each
+    // K stage waits for one memory round trip instead of two (F needs no
+    // indices: its gathered input sits in the slot copies).
     const int ncols = tl.col1 - tl.col0;
-    const bool cached = ncols <= IX;
+    const bool cached = PH == PH_B && ncols <= IX;
     if (cached) {
         for (int c = threadIdx.x; c < ncols; c += 256) {
             const int col = tl.col0 + c;
-            if (PH == PH_F) {
-                ix[2 * c] = ginv[2 * (nd.w + col)];
-                ix[2 * c + 1] = ginv[2 * (nd.w + col) + 1];
-            } else {
-                ix[c] = col >= s ? structs[nd.ustart + col - s] : -1;
-            }
+            ix[c] = col >= s ? structs[nd.ustart + col - s] : -1;
         }
     }
     __syncthreads();  // the cache is read by every thread's stage loads
@@ -853,15 +878,8 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
         };
         auto lb = [&](int col, int j) {
             if (j >= kc) return czero<T>();
-            if (!cached) return InnerOperand<T, PH>(nd, structs, ginv, Xp, W, col, r0 + j, k);
+            if (!cached) return InnerOperand<T, PH>(nd, structs, gs, Xp, W, col, r0 + j, k);
             const int c = col - tl.col0, rr = r0 + j;
-            if (PH == PH_F) {
-                cx<T> v = col < s ? Xp[(int64_t)(nd.off + col) * k + rr] : czero<T>();
-                const int g0 = ix[2 * c], g1 = ix[2 * c + 1];
-                if (g0 >= 0) v = v + W[(int64_t)g0 * k + rr];
-                if (g1 >= 0) v = v + W[(int64_t)g1 * k + rr];
-                return v;
-            }
             return col < s ? W[(nd.w + col) * k + rr] : Xp[(int64_t)ix[c] * k + rr];
         };
         typename Mfma<T>::acc acc[S::RB][S::CB];
@@ -871,7 +889,7 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
             if (tl.slot >= 0)
                 P[((int64_t)tl.slot * 64 + i) * k + r0 + j] = a;
             else
-                FinishGather<T, PH>(nd, ginv, Xp, W, tl.row0 + i, r0 + j, k, a);
+                FinishGather<T, PH>(nd, gs, Xp, W, tl.row0 + i, r0 + j, k, a);
         });
     }
 }
@@ -941,7 +959,7 @@ __global__ __launch_bounds__(256) void k_gemm_mma(const GemmDesc* descs, const G
 // phase's finish.  One workgroup per row block.
 template <typename T, int PH>
 __global__ __launch_bounds__(kThreads) void k_mv_reduce(const SolveTile* items, int start,
-                                                        const int32_t* ginv, cx<T>* Xp,
+                                                        Slots<T> gs, cx<T>* Xp,
                                                         cx<T>* W, const cx<T>* P, int k)
 {
     const SolveTile it = items[start + blockIdx.x];
@@ -950,7 +968,7 @@ __global__ __launch_bounds__(kThreads) void k_mv_reduce(const SolveTile* items, 
         const int rl = e / k, r = e % k;
         cx<T> a = czero<T>();
         for (int j = 0; j < it.col1; ++j) a = a + P[((int64_t)(it.col0 + j) * 64 + rl) * k + r];
-        FinishGather<T, PH>(nd, ginv, Xp, W, it.row0 + rl, r, k, a);
+        FinishGather<T, PH>(nd, gs, Xp, W, it.row0 + rl, r, k, a);
     }
 }
 
@@ -1362,14 +1380,16 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s)
 }
 
 template <typename T>
-void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, void* Pv, int k,
-                      hipStream_t s, MvProfile* prof, int onlyLevel, int onlyPhase)
+void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, void* Pv,
+                      void* Gv, int k, hipStream_t s, MvProfile* prof, int onlyLevel,
+                      int onlyPhase)
 {
     const cx<T>* L = (const cx<T>*)c.L + (int64_t)b * c.fstore;
     const cx<T>* LT = (const cx<T>*)c.LT + (int64_t)b * c.fstore;
     cx<T>* Xp = (cx<T>*)Xpv;
     cx<T>* W = (cx<T>*)Wv;
     cx<T>* P = (cx<T>*)Pv;
+    const Slots<T> gs{(cx<T>*)Gv, c.wstore * k, c.relinds};
     auto mv = [&](auto ph, const ClassPlan::SolveLevel& lv, const ClassPlan::Range& r) {
         if (r.count <= 0) return;
         constexpr int PH = decltype(ph)::value;
@@ -1377,18 +1397,18 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
         if (k >= 32) {
             if (r.rows == 64)
                 hipLaunchKernelGGL((k_mmw<T, PH, 64>), dim3(r.count), dim3(256), 0, s, c.mv,
-                                   r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+                                   r.start, c.structs, gs, L, LT, Xp, W, P, k);
             else if (r.rows == 32)
                 hipLaunchKernelGGL((k_mmw<T, PH, 32>), dim3(r.count), dim3(256), 0, s, c.mv,
-                                   r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+                                   r.start, c.structs, gs, L, LT, Xp, W, P, k);
             else
                 hipLaunchKernelGGL((k_mmw<T, PH, 16>), dim3(r.count), dim3(256), 0, s, c.mv,
-                                   r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+                                   r.start, c.structs, gs, L, LT, Xp, W, P, k);
         } else if (k >= 8) {
             auto launch = [&](auto rt, auto kb) {
                 constexpr int RT = decltype(rt)::value, KB = decltype(kb)::value;
                 hipLaunchKernelGGL((k_mvm<T, PH, RT, KB>), dim3(r.count), dim3(kMmThreads), 0, s,
-                                   c.mv, r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+                                   c.mv, r.start, c.structs, gs, L, LT, Xp, W, P, k);
             };
             auto byRows = [&](auto kb) {
                 if (r.rows == 64)
@@ -1404,13 +1424,13 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
                 byRows(std::integral_constant<int, 16>());
         } else if (r.rows == 64)
             hipLaunchKernelGGL((k_mv<T, PH, 64>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
-                               r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+                               r.start, c.structs, gs, L, LT, Xp, W, P, k);
         else if (r.rows == 32)
             hipLaunchKernelGGL((k_mv<T, PH, 32>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
-                               r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+                               r.start, c.structs, gs, L, LT, Xp, W, P, k);
         else
             hipLaunchKernelGGL((k_mv<T, PH, 16>), dim3(r.count), dim3(kMvThreads), 0, s, c.mv,
-                               r.start, c.structs, c.ginv, L, LT, Xp, W, P, k);
+                               r.start, c.structs, gs, L, LT, Xp, W, P, k);
         if (prof) {
             (void)hipEventRecord(prof->b, s);
             (void)hipEventSynchronize(prof->b);
@@ -1424,7 +1444,7 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
         }
         if (r.rcount > 0)
             hipLaunchKernelGGL((k_mv_reduce<T, PH>), dim3(r.rcount), dim3(kThreads), 0, s,
-                               c.mvred, r.rstart, c.ginv, Xp, W, P, k);
+                               c.mvred, r.rstart, gs, Xp, W, P, k);
     };
     if (onlyLevel >= 0) {  // one (level, phase) launch, for tile-shape tuning
         const ClassPlan::SolveLevel& lv = plan.solveLevels[onlyLevel];
@@ -1564,8 +1584,8 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
     template void LaunchLdlTrsm<T>(const ClassDev&, int, int, int, hipStream_t);                  \
     template void LaunchGemm<T>(const ClassDev&, int, int, hipStream_t);                          \
     template void LaunchInvDiag<T>(const ClassDev&, int, hipStream_t);                            \
-    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, void*, int,  \
-                                      hipStream_t, MvProfile*, int, int);                         \
+    template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, void*,   \
+                                      void*, int, hipStream_t, MvProfile*, int, int);             \
     template void LaunchTransposeScale<T>(const ClassDev&, hipStream_t);                          \
     template void LaunchFoldBack<T>(const ClassDev&, hipStream_t);                                \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \

@@ -36,7 +36,9 @@ struct StencilArgs {
 struct NodeDev {
     int64_t l, w;      // factor and work-vector offsets
     int32_t off, s, u; // first panel row, pivots, update size
-    int32_t ustart;    // into structs
+    int32_t ustart;    // into structs / relinds
+    int32_t pw;        // parent's work-vector offset (rows), 0 at a root
+    int32_t cslot;     // this node's child slot in its parent (0 or 1), -1 at a root
 };
 
 // A front-solve tile (or reduce item) with its node's fields inlined, so a
@@ -44,7 +46,7 @@ struct NodeDev {
 struct SolveTile {
     NodeDev nd;
     int32_t row0, nrows, col0, col1, slot;
-    int32_t pad_[3];
+    int32_t pad_;
 };
 static_assert(sizeof(SolveTile) == 64, "SolveTile is one 64-byte record");
 
@@ -107,7 +109,7 @@ struct MvProfile {
 // (n x k, k interleaved) with node work vectors W (wstore x k).
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xp, void* W,
-                      void* P,
+                      void* P, void* G,
                       int k, hipStream_t s, MvProfile* prof = nullptr, int onlyLevel = -1,
                       int onlyPhase = -1);
 template <typename T>

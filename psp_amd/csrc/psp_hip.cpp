@@ -54,6 +54,7 @@ struct psp_hip_ctx {
     std::vector<void*> facAllocs;  // Schur-complement and inversion scratch, only while factoring
     void *Xp = nullptr, *W = nullptr;
     void* P = nullptr;  // partial sums of split solve row blocks (64 rows x k per slot)
+    void* G[3] = {nullptr, nullptr, nullptr};  // per class: the children's update rows, two slot copies
     c128 *V = nullptr, *x = nullptr, *x0 = nullptr, *w = nullptr, *b = nullptr;
     double *partD = nullptr, *normD = nullptr;
     c128 *partC = nullptr, *Hcol = nullptr, *ydev = nullptr;
@@ -170,9 +171,17 @@ StencilArgs Stencil(const psp_hip_ctx* c)
 size_t FrontBytes(const psp_hip_ctx* c) { return c->desc.precision == PSP_HIP_FRONTS_FP32 ? 8 : 16; }
 
 // One-time device setup: upload the static plan and allocate factor storage.
-NodeDev MakeNodeDev(const Node& n)
+NodeDev MakeNodeDev(const ClassPlan& P, int t)
 {
-    return NodeDev{n.loff, n.woff, n.off, n.size, n.usize, (int32_t)n.ustart};
+    const Node& n = P.nodes[t];
+    int32_t pw = 0, cslot = -1;
+    if (n.parent >= 0) {
+        const Node& par = P.nodes[n.parent];
+        pw = (int32_t)par.woff;
+        for (int k = 0; k < 2; ++k)
+            if (par.child[k] == t) cslot = k;
+    }
+    return NodeDev{n.loff, n.woff, n.off, n.size, n.usize, (int32_t)n.ustart, pw, cslot};
 }
 
 std::vector<SolveTile> SolveTiles(const ClassPlan& P, const std::vector<MvTile>& tiles)
@@ -181,13 +190,13 @@ std::vector<SolveTile> SolveTiles(const ClassPlan& P, const std::vector<MvTile>&
     for (size_t i = 0; i < tiles.size(); ++i) {
         const MvTile& m = tiles[i];
         SolveTile& t = out[i];
-        t.nd = MakeNodeDev(P.nodes[m.node]);
+        t.nd = MakeNodeDev(P, m.node);
         t.row0 = m.row0;
         t.nrows = m.nrows;
         t.col0 = m.col0;
         t.col1 = m.col1;
         t.slot = m.slot;
-        t.pad_[0] = t.pad_[1] = t.pad_[2] = 0;
+        t.pad_ = 0;
     }
     return out;
 }
@@ -289,10 +298,7 @@ int EnsureDevice(psp_hip_ctx* c)
             (st = Upload(c, &D.panelVOff, vOff)))
             return st;
         std::vector<NodeDev> nd(D.nnodes);
-        for (int t = 0; t < D.nnodes; ++t) {
-            const Node& n = P.nodes[t];
-            nd[t] = MakeNodeDev(n);
-        }
+        for (int t = 0; t < D.nnodes; ++t) nd[t] = MakeNodeDev(P, t);
         if ((st = Upload(c, &D.nodes, nd))) return st;
         D.ntrans = (int)P.trans.size();
         D.nfold = (int)P.foldBack.size();
@@ -358,6 +364,15 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
     if ((st = Alloc(c, &c->Xp, eb * maxn * k, c->wsAllocs))) return st;
     if ((st = Alloc(c, &c->W, eb * maxw * k, c->wsAllocs))) return st;
     if ((st = Alloc(c, &c->P, eb * maxs * 64 * k, c->wsAllocs))) return st;
+    for (int cl = 0; cl < 3; ++cl) {
+        c->G[cl] = nullptr;
+        if (!c->plan.present[cl]) continue;
+        // Zero once: each class writes the same rows of its slot copies in
+        // every solve, and the rows it never writes must read as zero.
+        const size_t gb = eb * 2 * (size_t)c->plan.cls[cl].wstore * k;
+        if ((st = Alloc(c, &c->G[cl], gb, c->wsAllocs))) return st;
+        HIPCHK(c, hipMemsetAsync(c->G[cl], 0, gb, c->stream));
+    }
     if ((st = Alloc(c, &p, sizeof(c128) * N * k * m, c->wsAllocs))) return st;
     c->V = (c128*)p;
     if ((st = Alloc(c, &p, sizeof(c128) * N * k, c->wsAllocs))) return st;
@@ -501,7 +516,8 @@ void PanelForward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = nullp
     const ClassDev& D = c->dev[cl];
     hipStream_t s = c->stream;
     LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, k, s, prof);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, c->G[cl], k, s,
+                        prof);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s);
 }
 
@@ -519,7 +535,8 @@ void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = null
     const int v2 = g.panelV(p) + g.depth(p) - 1;
     const int vlLast = g.padding(p) + g.depth(p) - 1;
     LaunchSupdiagSet<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
-    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, k, s, prof);
+    LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, c->G[cl], k, s,
+                        prof);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 1, s);
 }
 
@@ -830,11 +847,11 @@ int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
             const size_t eb_ = FrontBytes(c);
             LaunchFillZero(c->Xp, (int64_t)(eb_ * P.n * k), s);
             LaunchFillZero(c->W, (int64_t)(eb_ * P.wstore * k), s);
-            LaunchPanelSolve<T>(D, P, 0, c->Xp, c->W, c->P, k, s, nullptr, d, ph);  // warm
+            LaunchPanelSolve<T>(D, P, 0, c->Xp, c->W, c->P, c->G[cl], k, s, nullptr, d, ph);  // warm
             constexpr int kReps = 3;
             HIPCHK(c, hipEventRecord(ea, s));
             for (int r = 0; r < kReps; ++r)
-                LaunchPanelSolve<T>(D, P, 0, c->Xp, c->W, c->P, k, s, nullptr, d, ph);
+                LaunchPanelSolve<T>(D, P, 0, c->Xp, c->W, c->P, c->G[cl], k, s, nullptr, d, ph);
             HIPCHK(c, hipEventRecord(eb, s));
             HIPCHK(c, hipEventSynchronize(eb));
             HIPCHK(c, hipGetLastError());
