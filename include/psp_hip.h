@@ -55,8 +55,11 @@ enum {
                                         hipGraph; 0: launch it kernel by kernel */
     PSP_HIP_OPT_SOLVE_MIN_TILES = 2, /* front-solve workgroups wanted per launch when
                                         choosing default tile shapes (default 256) */
-    PSP_HIP_OPT_SOLVE_MAX_SPLIT = 3  /* most inner slices per row block in the
+    PSP_HIP_OPT_SOLVE_MAX_SPLIT = 3, /* most inner slices per row block in the
                                         default tile shapes (default 8) */
+    PSP_HIP_OPT_SOLVE_WIDE_WG = 4    /* 1: default shapes give levels with few row
+                                        blocks 16-wave workgroups instead of inner
+                                        slices (8 <= k < 32); 0 (default): slices */
 };
 
 /* Flags for psp_hip_solve. */

@@ -19,6 +19,7 @@ DEVICE_POINTER = 1
 OPT_SWEEP_GRAPH = 1
 OPT_SOLVE_MIN_TILES = 2
 OPT_SOLVE_MAX_SPLIT = 3
+OPT_SOLVE_WIDE_WG = 4
 
 # Every entry point include/psp_hip.h declares.
 EXPORTS = (

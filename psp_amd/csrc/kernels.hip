@@ -544,8 +544,6 @@ struct Mfma<double> {
     static __device__ __forceinline__ int row(int lk, int q) { return lk + 4 * q; }
 };
 
-constexpr int kMmThreads = 256;  // 4 waves
-
 // Build-time tuning of the matrix-core solve kernel: factor loads in flight
 // per lane (FP32 / FP64 fronts) and the waves per SIMD the register
 // allocation must allow (0: the compiler's choice).
@@ -574,19 +572,23 @@ constexpr int kMmThreads = 256;  // 4 waves
 // for single-pass blocks the gathered bottom rows of F) are issued right
 // after the tile record arrives, so a workgroup waits for few memory round
 // trips.
-template <typename T, int PH, int RT, int KB>
-__global__ __launch_bounds__(kMmThreads) PSP_MVM_OCCUPANCY void k_mvm(const SolveTile* tiles, int start,
+template <typename T, int PH, int RT, int KB, int NWV>
+__global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveTile* tiles, int start,
                                                      const int32_t* structs,
                                                      Slots<T> gs, const cx<T>* L,
                                                      const cx<T>* LT, cx<T>* Xp, cx<T>* W,
                                                      cx<T>* P, int k)
 {
     typedef typename Mfma<T>::acc acc_t;
-    constexpr int NWV = kMmThreads / 64, WR = RT / 16, WK = NWV / WR, NB = KB / 8;
-    constexpr int CC = 2048 / KB, U = sizeof(T) == 4 ? PSP_MVM_U32 : PSP_MVM_U64;
+    constexpr int NT = 64 * NWV, WR = RT / 16, WK = NWV / WR, NB = KB / 8;
+    // factor loads in flight per lane; 16-wave workgroups (128 VGPRs a lane)
+    // keep fewer
+    constexpr int U = (sizeof(T) == 4 ? PSP_MVM_U32 : PSP_MVM_U64) / (NWV == 16 ? 4 : 1);
+    constexpr int CC = 2048 / KB;
     constexpr bool PRE = KB <= 16 && PH == PH_F;
-    static_assert(WR * WK == NWV && CC % 4 == 0, "tile shape");
-    static_assert(WK == 1 || 2 * CC * KB >= WR * NB * 256, "reduction scratch must fit in vs");
+    static_assert(WR * WK == NWV && CC % 4 == 0 && U >= 1, "tile shape");
+    static_assert(WK == 1 || 2 * CC * KB >= WR * (WK / 2) * NB * 256,
+                  "reduction scratch must fit in vs");
     const SolveTile tl = tiles[start + blockIdx.x];
     const NodeDev nd = tl.nd;
     const int s = nd.s, fs = nd.s + nd.u;
@@ -645,7 +647,7 @@ __global__ __launch_bounds__(kMmThreads) PSP_MVM_OCCUPANCY void k_mvm(const Solv
             load(m, 0);
             __syncthreads();  // the previous chunk's (or pass's) readers are done with vs
             const int nst = min(CC, 4 * nsteps);
-            for (int e = tid; e < nst * KB; e += kMmThreads) {
+            for (int e = tid; e < nst * KB; e += NT) {
                 const int c = e / KB, r = e % KB;
                 vs[c][r] = (r < kc && c < ncol)
                                ? InnerOperand<T, PH>(nd, structs, gs, Xp, W, cb + c, r0 + r, k)
@@ -675,23 +677,24 @@ __global__ __launch_bounds__(kMmThreads) PSP_MVM_OCCUPANCY void k_mvm(const Solv
                 }
             }
         }
-        // waves wk = 1 .. WK-1 hand their partial tiles to wave wk = 0, in order
+        // waves that share rows sum their partial tiles by a fixed tree
         if (WK > 1) {
-            for (int w = 1; w < WK; ++w) {
+            for (int h = WK / 2; h >= 1; h >>= 1) {
                 __syncthreads();
-                if (wk == w) {
-#pragma unroll
-                    for (int b = 0; b < NB; ++b)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) red[((wr * NB + b) * 64 + lane) * 4 + q] = acc[b][q];
-                }
-                __syncthreads();
-                if (wk == 0) {
+                if (wk >= h && wk < 2 * h) {
 #pragma unroll
                     for (int b = 0; b < NB; ++b)
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
-                            acc[b][q] += red[((wr * NB + b) * 64 + lane) * 4 + q];
+                            red[(((wr * h + wk - h) * NB + b) * 64 + lane) * 4 + q] = acc[b][q];
+                }
+                __syncthreads();
+                if (wk < h) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            acc[b][q] += red[(((wr * h + wk) * NB + b) * 64 + lane) * 4 + q];
                 }
             }
         }
@@ -1407,8 +1410,12 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
         } else if (k >= 8) {
             auto launch = [&](auto rt, auto kb) {
                 constexpr int RT = decltype(rt)::value, KB = decltype(kb)::value;
-                hipLaunchKernelGGL((k_mvm<T, PH, RT, KB>), dim3(r.count), dim3(kMmThreads), 0, s,
-                                   c.mv, r.start, c.structs, gs, L, LT, Xp, W, P, k);
+                if (r.waves == 16)
+                    hipLaunchKernelGGL((k_mvm<T, PH, RT, KB, 16>), dim3(r.count), dim3(1024), 0,
+                                       s, c.mv, r.start, c.structs, gs, L, LT, Xp, W, P, k);
+                else
+                    hipLaunchKernelGGL((k_mvm<T, PH, RT, KB, 4>), dim3(r.count), dim3(256), 0, s,
+                                       c.mv, r.start, c.structs, gs, L, LT, Xp, W, P, k);
             };
             auto byRows = [&](auto kb) {
                 if (r.rows == 64)

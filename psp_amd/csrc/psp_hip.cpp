@@ -867,16 +867,23 @@ int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
                 double tcur = 0;
                 if ((rc = timeShape(d, ph, cur, &tcur))) break;
                 double tbest = tcur;
+                std::vector<SolveShape> cands;
                 for (int rows : {64, 32, 16})
-                    for (int split : {1, 2, 4, 8}) {
-                        if (rc || (rows == cur.rows && split == cur.split)) continue;
-                        double t = 0;
-                        if ((rc = timeShape(d, ph, SolveShape{rows, split}, &t))) break;
-                        if (t < 0.97 * tbest) {
-                            tbest = t;
-                            best = SolveShape{rows, split};
-                        }
+                    for (int split : {1, 2, 4, 8}) cands.push_back(SolveShape{rows, split, 4});
+                if (k >= 8 && k < 32)  // 16-wave workgroups exist on the matrix-core path
+                    for (int rows : {32, 16})
+                        for (int split : {1, 2}) cands.push_back(SolveShape{rows, split, 16});
+                for (const SolveShape& cand : cands) {
+                    if (rc || (cand.rows == cur.rows && cand.split == cur.split &&
+                               cand.waves == cur.waves))
+                        continue;
+                    double t = 0;
+                    if ((rc = timeShape(d, ph, cand, &t))) break;
+                    if (t < 0.97 * tbest) {
+                        tbest = t;
+                        best = cand;
                     }
+                }
                 P.shapes[d][ph] = best;
                 tb += weight * tcur;
                 ta += weight * tbest;
@@ -952,8 +959,11 @@ int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
         c->useGraph = value != 0;
         return PSP_HIP_OK;
     }
-    if (option == PSP_HIP_OPT_SOLVE_MIN_TILES || option == PSP_HIP_OPT_SOLVE_MAX_SPLIT) {
-        if (option == PSP_HIP_OPT_SOLVE_MIN_TILES) {
+    if (option == PSP_HIP_OPT_SOLVE_MIN_TILES || option == PSP_HIP_OPT_SOLVE_MAX_SPLIT ||
+        option == PSP_HIP_OPT_SOLVE_WIDE_WG) {
+        if (option == PSP_HIP_OPT_SOLVE_WIDE_WG) {
+            c->plan.wideWg = value != 0;
+        } else if (option == PSP_HIP_OPT_SOLVE_MIN_TILES) {
             if (value < 1 || value > (1 << 20))
                 return Fail(c, PSP_HIP_ERR_INVALID, "solve min tiles out of range");
             c->plan.minTiles = value;
@@ -964,7 +974,7 @@ int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
         }
         for (int cl = 0; cl < 3; ++cl)
             if (c->plan.present[cl]) DefaultSolveShapes(c->plan.cls[cl], c->plan.minTiles,
-                                                        c->plan.maxSplit);
+                                                        c->plan.maxSplit, c->plan.wideWg);
         return RebuildSolveTiles(c);
     }
     return Fail(c, PSP_HIP_ERR_INVALID, "unknown option");

@@ -205,10 +205,13 @@ class DistHelmholtz:
             _raise(st, self._h, "Autotune")
         return b.value, a.value
 
-    def set_solve_tiling(self, min_tiles=None, max_split=None):
-        """Default front-solve tile shapes: workgroups wanted per launch and
-        the most inner slices per row block (rebuilds the solve tiles)."""
-        for opt, v in ((_lib.OPT_SOLVE_MIN_TILES, min_tiles), (_lib.OPT_SOLVE_MAX_SPLIT, max_split)):
+    def set_solve_tiling(self, min_tiles=None, max_split=None, wide_wg=None):
+        """Default front-solve tile shapes: workgroups wanted per launch, the
+        most inner slices per row block, and whether levels with few row
+        blocks get 16-wave workgroups instead of slices (rebuilds the solve
+        tiles)."""
+        for opt, v in ((_lib.OPT_SOLVE_MIN_TILES, min_tiles), (_lib.OPT_SOLVE_MAX_SPLIT, max_split),
+                       (_lib.OPT_SOLVE_WIDE_WG, wide_wg)):
             if v is None:
                 continue
             st = _lib.lib().psp_hip_set_option(self._ctx(), opt, int(v))

@@ -271,3 +271,22 @@ def test_autotuned_shapes_keep_parity(torch_cuda):
         h.Precondition(xd)
         torch.cuda.synchronize()
         assert _rel(xd.cpu().numpy()[l2n], o.precondition(X[l2n])) < 1e-10, k
+
+
+@pytest.mark.parametrize("k", [8, 13])
+def test_wide_workgroup_shapes_match_oracle(torch_cuda, k):
+    """16-wave workgroups (inner range spread over waves, summed by a fixed
+    LDS tree) instead of inner slices: panel solves and the sweep still
+    match the oracle."""
+    torch = torch_cuda
+    disc, vel = _problem(16, 16, 20)
+    o = _oracle(disc, vel, 5)
+    h = _helm(disc, vel, cutoff=5)
+    h.set_solve_tiling(min_tiles=100000, wide_wg=True)  # every level counts as "few tiles"
+    rng = np.random.default_rng(90 + k)
+    l2n = o.local_to_natural()
+    X = rand_block(rng, o.N, k)
+    xd = torch.from_numpy(X).cuda()
+    h.Precondition(xd)
+    torch.cuda.synchronize()
+    assert _rel(xd.cpu().numpy()[l2n], o.precondition(X[l2n])) < 1e-10
