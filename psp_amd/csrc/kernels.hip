@@ -321,6 +321,17 @@ __global__ __launch_bounds__(kThreads) void k_inv_diag(const InvItem* items, cx<
     }
 }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// operations but not for its outstanding global loads, so loads issued
+// before the barrier (the next operands) stay in flight across it.  Used
+// wherever a barrier only orders LDS accesses.
+__device__ __forceinline__ void LdsBarrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // --------------------------------------------------------------------------
 // Front solves on the padded panel vector Xp (rows x k, interleaved), with
 // the factors in the solve layout (rows < s of L: Linv with a stored unit
@@ -348,6 +359,21 @@ struct Slots {
     const int32_t* relinds;  // row in the parent's front of each update row
 };
 
+// Raw loads of a staged operand element, combined after they have arrived.
+template <typename T>
+struct Raw2 {
+    cx<T> a, b;
+};
+template <typename T>
+struct Raw3 {
+    cx<T> a, b, c;
+};
+template <typename T>
+struct RawRow {  // gathered value of a bottom row (two slot copies) and its parent row
+    cx<T> a, b;
+    int32_t prow;
+};
+
 // Forward input of front row r: the panel right-hand side on the node's own
 // rows plus the children's update vectors.
 template <typename T>
@@ -369,6 +395,70 @@ __device__ __forceinline__ cx<T> InnerOperand(const NodeDev& nd, const int32_t* 
     if (PH == PH_F) return Gathered(nd, gs, Xp, col, rr, k);
     if (col < nd.s) return W[(nd.w + col) * k + rr];
     return Xp[(int64_t)structs[nd.ustart + col - nd.s] * k + rr];
+}
+
+// The value v, passed through an empty asm: the compiler must hold it in a
+// register at this point, so a load that produced it is not sunk into the
+// arm of a later select (where its wait would sit inside a branch, one
+// round trip per element).
+template <class X>
+__device__ __forceinline__ X Opaque(X v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Branch-free staging of inner operands, in two steps so that a thread's
+// index loads (B: the struct rows of the update columns) are all in flight
+// together before the value loads that depend on them.  Every load is issued
+// unconditionally from a valid address (F: col < s, the forward inner range;
+// B: the struct list carries a sentinel entry, so structs[ustart] exists at a
+// root).
+template <int PH>
+__device__ __forceinline__ int32_t StageIndex(const NodeDev& nd, const int32_t* structs, int col)
+{
+    return PH == PH_F ? 0 : structs[nd.ustart + max(col - nd.s, 0)];
+}
+
+template <typename T, int PH>
+__device__ __forceinline__ cx<T> StageValue(const NodeDev& nd, const Slots<T>& gs, const cx<T>* Xp,
+                                            const cx<T>* W, int col, int rr, int k, int32_t srow)
+{
+    if (PH == PH_F) {
+        const int64_t g = (nd.w + col) * k + rr;
+        return gs.G[g] + gs.G[gs.half + g] + Xp[(int64_t)(nd.off + col) * k + rr];
+    }
+    const cx<T>* p = col < nd.s ? W + (nd.w + col) * k + rr : Xp + (int64_t)srow * k + rr;
+    return *p;
+}
+
+template <typename T, int PH>
+__device__ __forceinline__ cx<T> StageOperand(const NodeDev& nd, const int32_t* structs,
+                                              const Slots<T>& gs, const cx<T>* Xp,
+                                              const cx<T>* W, int col, int rr, int k)
+{
+    return StageValue<T, PH>(nd, gs, Xp, W, col, rr, k, Opaque(StageIndex<PH>(nd, structs, col)));
+}
+
+// N staged elements of a thread; at(q) gives element q's (inner index,
+// right-hand side) as an int2.
+template <typename T, int PH, int N, class AT>
+__device__ __forceinline__ void StageBatch(const NodeDev& nd, const int32_t* structs,
+                                           const Slots<T>& gs, const cx<T>* Xp, const cx<T>* W,
+                                           int k, const AT& at, cx<T> (&sv)[N])
+{
+    int32_t sx[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) sx[q] = StageIndex<PH>(nd, structs, at(q).x);
+    if (PH == PH_B) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) sx[q] = Opaque(sx[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const int2 p = at(q);
+        sv[q] = StageValue<T, PH>(nd, gs, Xp, W, p.x, p.y, k, sx[q]);
+    }
 }
 
 // Factor entry of phase PH at (row, col) of the node's block.
@@ -450,6 +540,14 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const SolveTile* tiles, int s
         for (int r = 0; r < KC; ++r) acc[r] = czero<T>();
         for (int cb = tl.col0; cb < tl.col1; cb += CC) {
             const int ncol = min(CC, tl.col1 - cb);
+            // staging loads first, then the factor loads, all in flight
+            // together; the LDS stores wait only for the staging loads
+            constexpr int SPT = CC * KC / kMvThreads;
+            cx<T> sv[SPT];
+            StageBatch<T, PH>(nd, structs, gs, Xp, W, k, [&](int q) {
+                const int e = tid + kMvThreads * q, c = e / KC, r = e % KC;
+                return int2{cb + min(c, ncol - 1), min(r0 + r, k - 1)};
+            }, sv);
             cx<T> m[NI];
 #pragma unroll
             for (int u = 0; u < NI; ++u) {
@@ -457,13 +555,13 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const SolveTile* tiles, int s
                 m[u] = (rowOk && c < ncol) ? FactorEntry<T, PH>(Lf, Lt, s, fs, row, cb + c)
                                            : czero<T>();
             }
-            __syncthreads();  // the previous chunk's readers are done with vs
-            for (int e = tid; e < ncol * KC; e += kMvThreads) {
-                const int c = e / KC, r = e % KC;
-                vs[c][r] = r < kc ? InnerOperand<T, PH>(nd, structs, gs, Xp, W, cb + c, r0 + r, k)
-                                  : czero<T>();
+            LdsBarrier();  // the previous chunk's readers are done with vs
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                const int e = tid + kMvThreads * q, c = e / KC, r = e % KC;
+                if (c < ncol) vs[c][r] = r < kc ? sv[q] : czero<T>();
             }
-            __syncthreads();
+            LdsBarrier();
 #pragma unroll
             for (int u = 0; u < NI; ++u) {
                 const int c = first + STRIDE * u;
@@ -487,12 +585,12 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const SolveTile* tiles, int s
 #pragma unroll
                 for (int r = 0; r < KC; ++r) red[wave - h][rl][r] = acc[r];
             }
-            __syncthreads();
+            LdsBarrier();
             if (wave < h && slot == 0) {
 #pragma unroll
                 for (int r = 0; r < KC; ++r) acc[r] = acc[r] + red[wave][rl][r];
             }
-            __syncthreads();
+            LdsBarrier();
         }
         if (wave == 0 && slot == 0 && rowOk) {
 #pragma unroll
@@ -607,22 +705,20 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
     cx<T> pre[PRE ? NB : 1][4];
     int prow[4] = {0, 0, 0, 0};
     if (PRE && fin && k <= KB) {
+        // branch-free: rows clamped into the node's front (the relinds list
+        // carries a sentinel entry), unused values dropped at the finish
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int row = tl.row0 + 16 * wr + Mfma<T>::row(lk, q);
-            if (row < tl.row0 + tl.nrows && row >= s) prow[q] = gs.relinds[nd.ustart + row - s];
+            const int row = min(tl.row0 + 16 * wr + Mfma<T>::row(lk, q), fs - 1);
+            prow[q] = gs.relinds[nd.ustart + max(row - s, 0)];
         }
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int row = tl.row0 + 16 * wr + Mfma<T>::row(lk, q), rr = 8 * b + li;
-                cx<T> g = czero<T>();
-                if (row < tl.row0 + tl.nrows && row >= s && rr < k) {
-                    const int64_t o = (nd.w + row) * k + rr;
-                    g = gs.G[o] + gs.G[gs.half + o];
-                }
-                pre[PRE ? b : 0][q] = g;
+                const int row = min(tl.row0 + 16 * wr + Mfma<T>::row(lk, q), fs - 1);
+                const int64_t o = (nd.w + row) * k + min(8 * b + li, k - 1);
+                pre[PRE ? b : 0][q] = gs.G[o] + gs.G[gs.half + o];
             }
     }
     for (int r0 = 0; r0 < k; r0 += KB) {
@@ -644,16 +740,25 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
                                  : czero<T>();
                 }
             };
-            load(m, 0);
-            __syncthreads();  // the previous chunk's (or pass's) readers are done with vs
+            // staging loads first, then the first factor batch, all in flight
+            // together; the LDS stores wait only for the staging loads
+            // (columns up to the next multiple of 4 are written, as zeros
+            // past the range)
+            constexpr int SPT = CC * KB / NT;
             const int nst = min(CC, 4 * nsteps);
-            for (int e = tid; e < nst * KB; e += NT) {
-                const int c = e / KB, r = e % KB;
-                vs[c][r] = (r < kc && c < ncol)
-                               ? InnerOperand<T, PH>(nd, structs, gs, Xp, W, cb + c, r0 + r, k)
-                               : czero<T>();
+            cx<T> sv[SPT];
+            StageBatch<T, PH>(nd, structs, gs, Xp, W, k, [&](int q) {
+                const int e = tid + NT * q, c = e / KB, r = e % KB;
+                return int2{cb + min(c, ncol - 1), min(r0 + r, k - 1)};
+            }, sv);
+            load(m, 0);
+            LdsBarrier();  // the previous chunk's (or pass's) readers are done with vs
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                const int e = tid + NT * q, c = e / KB, r = e % KB;
+                if (c < nst) vs[c][r] = (r < kc && c < ncol) ? sv[q] : czero<T>();
             }
-            __syncthreads();
+            LdsBarrier();
             for (int u0 = 0; u0 < mine; u0 += U) {
                 cx<T> mn[U];
                 if (u0 + U < mine) load(mn, u0 + U);
@@ -680,7 +785,7 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
         // waves that share rows sum their partial tiles by a fixed tree
         if (WK > 1) {
             for (int h = WK / 2; h >= 1; h >>= 1) {
-                __syncthreads();
+                LdsBarrier();
                 if (wk >= h && wk < 2 * h) {
 #pragma unroll
                     for (int b = 0; b < NB; ++b)
@@ -688,7 +793,7 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
                         for (int q = 0; q < 4; ++q)
                             red[(((wr * h + wk - h) * NB + b) * 64 + lane) * 4 + q] = acc[b][q];
                 }
-                __syncthreads();
+                LdsBarrier();
                 if (wk < h) {
 #pragma unroll
                     for (int b = 0; b < NB; ++b)
@@ -734,25 +839,34 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
 // A and B are read through policy objects: a(i, k) and b(k, j) return the
 // complex entry or zero outside the operand.
 // --------------------------------------------------------------------------
-template <int RT>
+template <int RT, int CT = 64>
 struct TileShape {
-    static constexpr int WRW = RT == 64 ? 2 : 1;       // waves along rows
+    static constexpr int WRW = RT >= 64 ? 2 : 1;       // waves along rows
     static constexpr int WCW = 4 / WRW;                // waves along columns
     static constexpr int RB = RT / 16 / WRW;           // 16-row blocks per wave
-    static constexpr int CB = 8 / WCW;                 // 8-column blocks per wave
+    static constexpr int CB = CT / 8 / WCW;            // 8-column blocks per wave
 };
 
 constexpr int kBK = 16;  // inner columns per LDS stage (solve tiles)
 
-template <typename T, int RT, bool BJFAST, int BK = kBK, class LA, class LB>
-__device__ __forceinline__ void MmaTile(const LA& la, const LB& lb, int kbeg, int kend,
-                                        cx<T> (*As)[BK][RT + 1], cx<T> (*Bs)[BK][65],
-                                        typename Mfma<T>::acc (&acc)[TileShape<RT>::RB]
-                                                                   [TileShape<RT>::CB])
+// la(i, k) / lb(k, j) return the raw loads of one A / B element and must be
+// safe to call for every i < RT, j < CT and k in [kbeg, kend) (loaders clamp
+// their own indices, so a stage's loads are issued without branches);
+// fa(i, k, raw) / fb(k, j, raw) turn them into the operand value, masks and
+// scaling included.  The transforms run when the next stage is written to
+// LDS, after this stage's matrix-core steps, so the loads of stage s + 1 are
+// in flight while stage s computes.
+template <typename T, int RT, bool BJFAST, int BK = kBK, int CT = 64, class LA, class FA, class LB,
+          class FB>
+__device__ __forceinline__ void MmaTile(const LA& la, const FA& fa, const LB& lb, const FB& fb,
+                                        int kbeg, int kend, cx<T> (*As)[BK][RT + 1],
+                                        cx<T> (*Bs)[BK][CT + 1],
+                                        typename Mfma<T>::acc (&acc)[TileShape<RT, CT>::RB]
+                                                                   [TileShape<RT, CT>::CB])
 {
-    typedef TileShape<RT> S;
+    typedef TileShape<RT, CT> S;
     constexpr int kBK = BK;
-    constexpr int NA = RT * kBK / 256, NBL = 64 * kBK / 256;  // loads per thread per stage
+    constexpr int NA = RT * kBK / 256, NBL = CT * kBK / 256;  // loads per thread per stage
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave / S::WCW, wc = wave % S::WCW;
     const int li = lane & 15, lk = lane >> 4;
@@ -761,38 +875,39 @@ __device__ __forceinline__ void MmaTile(const LA& la, const LB& lb, int kbeg, in
 #pragma unroll
         for (int b = 0; b < S::CB; ++b) acc[a][b] = typename Mfma<T>::acc{0, 0, 0, 0};
     if (kend <= kbeg) return;
-    cx<T> ra[NA], rb[NBL];
+    decltype(la(0, kbeg)) ra[NA];
+    decltype(lb(kbeg, 0)) rb[NBL];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
             const int e = tid + 256 * q, i = e % RT, kk = e / RT;
-            ra[q] = k0 + kk < kend ? la(i, k0 + kk) : czero<T>();
+            ra[q] = la(i, min(k0 + kk, kend - 1));
         }
 #pragma unroll
         for (int q = 0; q < NBL; ++q) {
             const int e = tid + 256 * q;
-            const int j = BJFAST ? e % 64 : e / kBK, kk = BJFAST ? e / 64 : e % kBK;
-            rb[q] = k0 + kk < kend ? lb(k0 + kk, j) : czero<T>();
+            const int j = BJFAST ? e % CT : e / kBK, kk = BJFAST ? e / CT : e % kBK;
+            rb[q] = lb(min(k0 + kk, kend - 1), j);
         }
     };
-    auto put = [&](int buf) {
+    auto put = [&](int buf, int k0) {
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
-            const int e = tid + 256 * q;
-            As[buf][e / RT][e % RT] = ra[q];
+            const int e = tid + 256 * q, i = e % RT, kk = e / RT;
+            As[buf][kk][i] = k0 + kk < kend ? fa(i, k0 + kk, ra[q]) : czero<T>();
         }
 #pragma unroll
         for (int q = 0; q < NBL; ++q) {
             const int e = tid + 256 * q;
-            const int j = BJFAST ? e % 64 : e / kBK, kk = BJFAST ? e / 64 : e % kBK;
-            Bs[buf][kk][j] = rb[q];
+            const int j = BJFAST ? e % CT : e / kBK, kk = BJFAST ? e / CT : e % kBK;
+            Bs[buf][kk][j] = k0 + kk < kend ? fb(k0 + kk, j, rb[q]) : czero<T>();
         }
     };
     const int nst = (kend - kbeg + kBK - 1) / kBK;
     fetch(kbeg);
-    __syncthreads();  // the caller's previous use of As / Bs is over
-    put(0);
-    __syncthreads();
+    LdsBarrier();  // the caller's previous use of As / Bs is over
+    put(0, kbeg);
+    LdsBarrier();
     for (int st = 0; st < nst; ++st) {
         const int buf = st & 1;
         if (st + 1 < nst) fetch(kbeg + (st + 1) * kBK);
@@ -813,19 +928,54 @@ __device__ __forceinline__ void MmaTile(const LA& la, const LB& lb, int kbeg, in
                 }
             }
         }
-        if (st + 1 < nst) put(buf ^ 1);
-        __syncthreads();
+        if (st + 1 < nst) put(buf ^ 1, kbeg + (st + 1) * kBK);
+        LdsBarrier();
+    }
+}
+
+// Epilogue over the tile elements this lane finishes, one 16-row block at a
+// time: ld(i, j) is called for every element of the block first (it must be
+// safe for any i < RT, j < CT: the loads of the block are all in flight
+// together), then st(i, j, value, loaded) for the elements of lanes with
+// l % 16 < 8, after pairing the real and imaginary halves.
+template <typename T, int RT, int CT = 64, class LD, class ST>
+__device__ __forceinline__ void MmaEpilogue(const typename Mfma<T>::acc (&acc)[TileShape<RT, CT>::RB]
+                                                                             [TileShape<RT, CT>::CB],
+                                            const LD& ld, const ST& st)
+{
+    typedef TileShape<RT, CT> S;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = wave / S::WCW, wc = wave % S::WCW;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < S::RB; ++a) {
+        decltype(ld(0, 0)) old[S::CB][4];
+#pragma unroll
+        for (int b = 0; b < S::CB; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                old[b][q] = ld(16 * (wr * S::RB + a) + Mfma<T>::row(lk, q), 8 * (wc * S::CB + b) + (li & 7));
+#pragma unroll
+        for (int b = 0; b < S::CB; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const T val = acc[a][b][q];
+                const T other = __shfl_xor(val, 8, 64);
+                if (li < 8)
+                    st(16 * (wr * S::RB + a) + Mfma<T>::row(lk, q), 8 * (wc * S::CB + b) + li,
+                       cx<T>{val, other}, old[b][q]);
+            }
     }
 }
 
 // Calls f(i, j, value) for every tile element this lane finishes (lanes with
 // l % 16 < 8 after pairing the real and imaginary halves).
-template <typename T, int RT, class F>
-__device__ __forceinline__ void MmaForEach(const typename Mfma<T>::acc (&acc)[TileShape<RT>::RB]
-                                                                            [TileShape<RT>::CB],
+template <typename T, int RT, int CT = 64, class F>
+__device__ __forceinline__ void MmaForEach(const typename Mfma<T>::acc (&acc)[TileShape<RT, CT>::RB]
+                                                                            [TileShape<RT, CT>::CB],
                                            const F& f)
 {
-    typedef TileShape<RT> S;
+    typedef TileShape<RT, CT> S;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = wave / S::WCW, wc = wave % S::WCW;
     const int li = lane & 15, lk = lane >> 4;
@@ -869,31 +1019,73 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
     if (cached) {
         for (int c = threadIdx.x; c < ncols; c += 256) {
             const int col = tl.col0 + c;
-            ix[c] = col >= s ? structs[nd.ustart + col - s] : -1;
+            ix[c] = col >= s ? structs[nd.ustart + col - s] : col;
         }
     }
-    __syncthreads();  // the cache is read by every thread's stage loads
+    LdsBarrier();  // the cache is read by every thread's stage loads
+    const int ilast = tl.nrows - 1;
+    // A: the factor block, rows clamped into the tile, triangle masked later.
+    auto la = [&](int i, int col) {
+        const int row = tl.row0 + min(i, ilast);
+        return PH == PH_F ? Lf[row + (int64_t)col * fs] : Lt[row + (int64_t)col * s];
+    };
+    auto fa = [&](int i, int col, cx<T> v) {
+        const int row = tl.row0 + i;
+        const bool in = i < tl.nrows && (PH == PH_F ? col <= row : col >= row);
+        return in ? v : czero<T>();
+    };
     for (int r0 = 0; r0 < k; r0 += 64) {
         const int kc = min(64, k - r0);
-        auto la = [&](int i, int col) {
-            return i < tl.nrows ? FactorEntry<T, PH>(Lf, Lt, s, fs, tl.row0 + i, col)
-                                : czero<T>();
-        };
-        auto lb = [&](int col, int j) {
-            if (j >= kc) return czero<T>();
-            if (!cached) return InnerOperand<T, PH>(nd, structs, gs, Xp, W, col, r0 + j, k);
-            const int c = col - tl.col0, rr = r0 + j;
-            return col < s ? W[(nd.w + col) * k + rr] : Xp[(int64_t)ix[c] * k + rr];
-        };
         typename Mfma<T>::acc acc[S::RB][S::CB];
-        MmaTile<T, RT, true>(la, lb, tl.col0, tl.col1, As, Bs, acc);
-        MmaForEach<T, RT>(acc, [&](int i, int j, cx<T> a) {
-            if (i >= tl.nrows || j >= kc) return;
-            if (tl.slot >= 0)
-                P[((int64_t)tl.slot * 64 + i) * k + r0 + j] = a;
-            else
-                FinishGather<T, PH>(nd, gs, Xp, W, tl.row0 + i, r0 + j, k, a);
-        });
+        if constexpr (PH == PH_F) {
+            // B: the gathered forward input (slot copies plus the panel rows).
+            auto lb = [&](int col, int j) {
+                const int rr = r0 + min(j, kc - 1);
+                const int64_t g = (nd.w + col) * k + rr;
+                return Raw3<T>{gs.G[g], gs.G[gs.half + g], Xp[(int64_t)(nd.off + col) * k + rr]};
+            };
+            auto fb = [&](int, int j, const Raw3<T>& v) { return j < kc ? v.a + v.b + v.c : czero<T>(); };
+            MmaTile<T, RT, true>(la, fa, lb, fb, tl.col0, tl.col1, As, Bs, acc);
+        } else {
+            auto fb = [&](int, int j, cx<T> v) { return j < kc ? v : czero<T>(); };
+            if (cached) {
+                auto lb = [&](int col, int j) {
+                    const int rr = r0 + min(j, kc - 1);
+                    // ix holds the W row (top columns, plus nd.w) or the
+                    // panel row: the read feeds both arms, so no branch
+                    const int64_t row = (int64_t)ix[col - tl.col0] + (col < s ? nd.w : 0);
+                    return (col < s ? W : Xp)[row * k + rr];
+                };
+                MmaTile<T, RT, true>(la, fa, lb, fb, tl.col0, tl.col1, As, Bs, acc);
+            } else {
+                auto lb = [&](int col, int j) {
+                    return StageOperand<T, PH>(nd, structs, gs, Xp, W, col, r0 + min(j, kc - 1), k);
+                };
+                MmaTile<T, RT, true>(la, fa, lb, fb, tl.col0, tl.col1, As, Bs, acc);
+            }
+        }
+        if (tl.slot >= 0) {
+            MmaForEach<T, RT>(acc, [&](int i, int j, cx<T> a) {
+                if (i < tl.nrows && j < kc) P[((int64_t)tl.slot * 64 + i) * k + r0 + j] = a;
+            });
+        } else if constexpr (PH == PH_F) {
+            // Bottom rows add their gathered value; its loads (and the parent
+            // row of each) are issued for a whole row block before any store.
+            auto ld = [&](int i, int j) {
+                const int row = tl.row0 + min(i, ilast);
+                const int64_t g = (nd.w + row) * k + r0 + min(j, kc - 1);
+                return RawRow<T>{gs.G[g], gs.G[gs.half + g],
+                                 gs.relinds[nd.ustart + max(row - s, 0)]};
+            };
+            MmaEpilogue<T, RT>(acc, ld, [&](int i, int j, cx<T> a, const RawRow<T>& g) {
+                if (i < tl.nrows && j < kc)
+                    Finish<T, PH>(nd, gs, Xp, W, tl.row0 + i, r0 + j, k, a, g.a + g.b, g.prow);
+            });
+        } else {
+            MmaForEach<T, RT>(acc, [&](int i, int j, cx<T> a) {
+                if (i < tl.nrows && j < kc) Finish<T, PH>(nd, gs, Xp, W, tl.row0 + i, r0 + j, k, a, czero<T>(), 0);
+            });
+        }
     }
 }
 
@@ -906,8 +1098,9 @@ __global__ __launch_bounds__(256) void k_gemm_mma(const GemmDesc* descs, const G
                                                   int64_t st2, cx<T>* b3)
 {
     constexpr int BK = kBK;  // inner columns per stage (32 measured slower on the factor)
-    __shared__ cx<T> As[2][BK][65];
-    __shared__ cx<T> Bs[2][BK][65];
+    constexpr int TS = 64;   // 128x128 tiles measured slower on the Schur updates
+    __shared__ cx<T> As[2][BK][TS + 1];
+    __shared__ cx<T> Bs[2][BK][TS + 1];
     const GemmTile tl = tiles[tileStart + blockIdx.x];
     const GemmDesc g = descs[tl.desc];
     const int bt = blockIdx.y;
@@ -920,42 +1113,58 @@ __global__ __launch_bounds__(256) void k_gemm_mma(const GemmDesc* descs, const G
     const int tm = tl.tm, tn = tl.tn, f = g.flags;
     int kbeg = 0, kend = g.K;
     if (f & G_BUNIT) kbeg = tn;
-    if (f & G_AUNIT) kend = min(kend, tm + 64);
+    if (f & G_AUNIT) kend = min(kend, tm + TS);
+    // Every loader reads from a clamped index inside the operand's rectangle
+    // (always allocated), so a stage's loads go out without branches; the
+    // masks, the unit diagonals and the diag(D) scaling are applied when the
+    // stage is written to LDS.  Without G_HASD the D load reads A's first
+    // element (a broadcast) and is ignored.
+    const bool hasd = f & G_HASD;
+    const cx<T>* Dv = hasd ? D : A;
+    const int64_t dinc = hasd ? g.dinc : 0;
     auto la = [&](int i, int k) {
+        return Raw2<T>{A[min(tm + i, g.M - 1) + (int64_t)k * g.lda], Dv[(int64_t)k * dinc]};
+    };
+    auto fa = [&](int i, int k, const Raw2<T>& r) {
         const int gi = tm + i;
-        if (gi >= g.M) return czero<T>();
-        cx<T> v;
-        if (f & G_AUNIT)
-            v = gi > k ? A[gi + (int64_t)k * g.lda] : (gi == k ? cx<T>{T(1), T(0)} : czero<T>());
-        else
-            v = A[gi + (int64_t)k * g.lda];
-        if (f & G_HASD) v = v * D[(int64_t)k * g.dinc];
-        return v;
+        cx<T> v = r.a;
+        if (f & G_AUNIT) v = gi > k ? v : (gi == k ? cx<T>{T(1), T(0)} : czero<T>());
+        if (hasd) v = v * r.b;
+        return gi < g.M ? v : czero<T>();
     };
-    auto lbt = [&](int k, int j) {
-        const int gj = tn + j;
-        return gj < g.N ? B[gj + (int64_t)k * g.ldb] : czero<T>();
+    typename Mfma<T>::acc acc[TileShape<TS, TS>::RB][TileShape<TS, TS>::CB];
+    if (f & G_BTRANS) {
+        auto lb = [&](int k, int j) { return B[min(tn + j, g.N - 1) + (int64_t)k * g.ldb]; };
+        auto fb = [&](int, int j, cx<T> v) { return tn + j < g.N ? v : czero<T>(); };
+        MmaTile<T, TS, true, BK, TS>(la, fa, lb, fb, kbeg, kend, As, Bs, acc);
+    } else {
+        auto lb = [&](int k, int j) { return B[k + (int64_t)min(tn + j, g.N - 1) * g.ldb]; };
+        auto fb = [&](int k, int j, cx<T> v) {
+            const int gj = tn + j;
+            if (f & G_BUNIT) v = k > gj ? v : (k == gj ? cx<T>{T(1), T(0)} : czero<T>());
+            return gj < g.N ? v : czero<T>();
+        };
+        MmaTile<T, TS, false, BK, TS>(la, fa, lb, fb, kbeg, kend, As, Bs, acc);
+    }
+    auto offset = [&](int gi, int gj) {
+        return (f & G_CTRANS) ? gj + (int64_t)gi * g.ldc : gi + (int64_t)gj * g.ldc;
     };
-    auto lbn = [&](int k, int j) {
-        const int gj = tn + j;
-        if (gj >= g.N) return czero<T>();
-        if (f & G_BUNIT)
-            return k > gj ? B[k + (int64_t)gj * g.ldb] : (k == gj ? cx<T>{T(1), T(0)} : czero<T>());
-        return B[k + (int64_t)gj * g.ldb];
+    auto keep = [&](int gi, int gj) {
+        return gi < g.M && gj < g.N && !((f & G_CLOWER) && gi < gj);
     };
-    typename Mfma<T>::acc acc[TileShape<64>::RB][TileShape<64>::CB];
-    if (f & G_BTRANS)
-        MmaTile<T, 64, true, BK>(la, lbt, kbeg, kend, As, Bs, acc);
-    else
-        MmaTile<T, 64, false, BK>(la, lbn, kbeg, kend, As, Bs, acc);
-    MmaForEach<T, 64>(acc, [&](int i, int j, cx<T> v) {
-        const int gi = tm + i, gj = tn + j;
-        if (gi >= g.M || gj >= g.N) return;
-        if ((f & G_CLOWER) && gi < gj) return;
-        cx<T>* c = (f & G_CTRANS) ? Cm + gj + (int64_t)gi * g.ldc : Cm + gi + (int64_t)gj * g.ldc;
-        if (f & G_NEG) v = -v;
-        *c = (f & G_BETA0) ? v : (*c + v);
-    });
+    if (f & G_BETA0) {
+        MmaForEach<T, TS, TS>(acc, [&](int i, int j, cx<T> v) {
+            const int gi = tm + i, gj = tn + j;
+            if (keep(gi, gj)) Cm[offset(gi, gj)] = (f & G_NEG) ? -v : v;
+        });
+    } else {
+        // C is read for a whole row block before the first store goes out.
+        auto ld = [&](int i, int j) { return Cm[offset(min(tm + i, g.M - 1), min(tn + j, g.N - 1))]; };
+        MmaEpilogue<T, TS, TS>(acc, ld, [&](int i, int j, cx<T> v, cx<T> c) {
+            const int gi = tm + i, gj = tn + j;
+            if (keep(gi, gj)) Cm[offset(gi, gj)] = c + ((f & G_NEG) ? -v : v);
+        });
+    }
 }
 
 // Sum of the inner slices of split row blocks, in slot order, then the
@@ -1367,10 +1576,9 @@ template <typename T>
 void LaunchGemm(const ClassDev& c, int tileStart, int tileCount, hipStream_t s)
 {
     if (tileCount <= 0) return;
-    cx<T>* T0 = (cx<T>*)c.T;
     hipLaunchKernelGGL((k_gemm_mma<T>), dim3(tileCount, c.npanels), dim3(256), 0, s, c.gemms,
-                       c.tiles, tileStart, (cx<T>*)c.L, c.fstore, (cx<T>*)c.U, c.ustore, T0,
-                       c.tstore, (cx<T>*)c.LT);
+                       c.tiles, tileStart, (cx<T>*)c.L, c.fstore, (cx<T>*)c.U, c.ustore,
+                       (cx<T>*)c.T, c.tstore, (cx<T>*)c.LT);
 }
 
 template <typename T>

@@ -171,6 +171,15 @@ StencilArgs Stencil(const psp_hip_ctx* c)
 size_t FrontBytes(const psp_hip_ctx* c) { return c->desc.precision == PSP_HIP_FRONTS_FP32 ? 8 : 16; }
 
 // One-time device setup: upload the static plan and allocate factor storage.
+// The struct list with one trailing entry, so that a branch-free kernel
+// may read structs[ustart] of a front without an update (the root).
+std::vector<int32_t> WithSentinel(const std::vector<int>& v)
+{
+    std::vector<int32_t> out(v.begin(), v.end());
+    out.push_back(0);
+    return out;
+}
+
 NodeDev MakeNodeDev(const ClassPlan& P, int t)
 {
     const Node& n = P.nodes[t];
@@ -287,8 +296,8 @@ int EnsureDevice(psp_hip_ctx* c)
             (st = Upload(c, &D.qrows, P.qrows)) || (st = Upload(c, &D.nat2q, nat2q)) ||
             (st = Upload(c, &D.nodeL, nL)) || (st = Upload(c, &D.nodeW, nW)) ||
             (st = Upload(c, &D.nodeS, nS)) || (st = Upload(c, &D.nodeU, nU)) ||
-            (st = Upload(c, &D.nodeUs, nUs)) || (st = Upload(c, &D.structs, P.structs)) ||
-            (st = Upload(c, &D.relinds, P.relinds)) || (st = Upload(c, &D.extadd, P.extadd)) ||
+            (st = Upload(c, &D.nodeUs, nUs)) || (st = Upload(c, &D.structs, WithSentinel(P.structs))) ||
+            (st = Upload(c, &D.relinds, WithSentinel(P.relinds))) || (st = Upload(c, &D.extadd, P.extadd)) ||
             (st = Upload(c, &D.ldl, P.ldl)) || (st = Upload(c, &D.gemms, P.gemms)) ||
             (st = Upload(c, &D.tiles, P.tiles)) || (st = Upload(c, &D.inv, P.inv)) ||
             (st = Upload(c, &D.mv, SolveTiles(P, P.mv))) ||
