@@ -57,9 +57,17 @@ enum {
                                         choosing default tile shapes (default 256) */
     PSP_HIP_OPT_SOLVE_MAX_SPLIT = 3, /* most inner slices per row block in the
                                         default tile shapes (default 8) */
-    PSP_HIP_OPT_SOLVE_WIDE_WG = 4    /* 1: default shapes give levels with few row
+    PSP_HIP_OPT_SOLVE_WIDE_WG = 4,   /* 1: default shapes give levels with few row
                                         blocks 16-wave workgroups instead of inner
                                         slices (8 <= k < 32); 0 (default): slices */
+    PSP_HIP_OPT_SOLVE_PERSIST = 5,   /* 1 (default): default shapes run the front
+                                        solves with persistent, pipelined
+                                        workgroups (8 <= k <= 16); 0: one
+                                        workgroup per tile */
+    PSP_HIP_OPT_SOLVE_PERSIST_GRID = 6 /* persistent launches use at most this many
+                                          workgroups (more only when one would
+                                          get over 32 tiles); 0 (default): as many
+                                          as fit on the device at once */
 };
 
 /* Flags for psp_hip_solve. */

@@ -20,6 +20,8 @@ OPT_SWEEP_GRAPH = 1
 OPT_SOLVE_MIN_TILES = 2
 OPT_SOLVE_MAX_SPLIT = 3
 OPT_SOLVE_WIDE_WG = 4
+OPT_SOLVE_PERSIST = 5
+OPT_SOLVE_PERSIST_GRID = 6
 
 # Every entry point include/psp_hip.h declares.
 EXPORTS = (

@@ -408,6 +408,17 @@ __device__ __forceinline__ X Opaque(X v)
     return v;
 }
 
+// An empty asm that reads v: every load that may still write v is waited for
+// here, so its register is free of pending writes on every path that
+// follows (without it, a register whose load result is used on one branch
+// only can be reused on the other while the load is still in flight, and
+// the reuse then waits for every outstanding load).
+template <class X>
+__device__ __forceinline__ void Sink(const X& v)
+{
+    asm volatile("" ::"v"(v));
+}
+
 // Branch-free staging of inner operands, in two steps so that a thread's
 // index loads (B: the struct rows of the update columns) are all in flight
 // together before the value loads that depend on them.  Every load is issued
@@ -646,10 +657,19 @@ struct Mfma<double> {
 // per lane (FP32 / FP64 fronts) and the waves per SIMD the register
 // allocation must allow (0: the compiler's choice).
 #ifndef PSP_MVM_U32
-#define PSP_MVM_U32 16
+#define PSP_MVM_U32 8
 #endif
 #ifndef PSP_MVM_U64
 #define PSP_MVM_U64 8
+#endif
+// Staged elements (columns x right-hand sides) per chunk, by front precision:
+// smaller chunks keep fewer staging registers, so more workgroups fit on a
+// CU (each launch then runs in fewer rounds of workgroups).
+#ifndef PSP_MVM_CHUNK32
+#define PSP_MVM_CHUNK32 1024
+#endif
+#ifndef PSP_MVM_CHUNK64
+#define PSP_MVM_CHUNK64 512
 #endif
 #ifndef PSP_MVM_WAVES_PER_EU
 #define PSP_MVM_WAVES_PER_EU 0
@@ -682,7 +702,9 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
     // factor loads in flight per lane; 16-wave workgroups (128 VGPRs a lane)
     // keep fewer
     constexpr int U = (sizeof(T) == 4 ? PSP_MVM_U32 : PSP_MVM_U64) / (NWV == 16 ? 4 : 1);
-    constexpr int CC = 2048 / KB;
+    // staged inner columns per chunk; 16-wave tiles reuse the chunk buffer
+    // as reduction scratch and keep the large one
+    constexpr int CC = (NWV == 16 ? 2048 : sizeof(T) == 4 ? PSP_MVM_CHUNK32 : PSP_MVM_CHUNK64) / KB;
     constexpr bool PRE = KB <= 16 && PH == PH_F;
     static_assert(WR * WK == NWV && CC % 4 == 0 && U >= 1, "tile shape");
     static_assert(WK == 1 || 2 * CC * KB >= WR * (WK / 2) * NB * 256,
@@ -825,6 +847,302 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
                     }
                 }
             }
+        }
+    }
+}
+
+// Persistent, pipelined matrix-core path (8 <= k <= KB, KB = 8 or 16: one
+// pass over the right-hand sides).  gridDim.x workgroups take the launch's
+// tiles round-robin (their descriptors are read into LDS once) and work
+// through them as a sequence of steps of CW inner columns.  Two register
+// sets alternate: while step n is multiplied, the staging and factor loads
+// of step n + 1 are in flight, and in the backward phase also the struct
+// rows that the gather of step n + 2 needs.  The products, the summation
+// order within a tile (4-column steps in increasing order, the waves of a
+// row block summed by the same fixed tree) and the finish are those of
+// k_mvm; only the schedule differs.  No workgroup waits for another.
+constexpr int kMvpTiles = 32;  // tiles per workgroup at most (host sizes the grid)
+#ifndef PSP_MVP_U
+#define PSP_MVP_U 8  // factor loads per lane per step, 64- and 32-row tiles
+#endif
+#ifndef PSP_MVP_U16
+#define PSP_MVP_U16 4  // factor loads per lane per step, 16-row tiles
+#endif
+
+template <typename T, int PH, int RT, int KB>
+struct MvpShape {
+    static constexpr int NT = 256, WR = RT / 16, WK = 4 / WR, NB = KB / 8;
+    static constexpr int U = RT == 16 ? PSP_MVP_U16 : PSP_MVP_U;  // factor loads per lane per step
+    static constexpr int CW = 4 * WK * U;        // inner columns per step
+    static constexpr int SPT = CW * KB / NT;     // staged elements per thread per step
+    static_assert(CW * KB == SPT * NT && SPT >= 1, "step shape");
+};
+
+template <typename T, int PH, int RT, int KB>
+__global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, int count,
+                                             const int32_t* structs, Slots<T> gs,
+                                             const cx<T>* L, const cx<T>* LT, cx<T>* Xp,
+                                             cx<T>* W, cx<T>* P, int k)
+{
+    typedef typename Mfma<T>::acc acc_t;
+    typedef MvpShape<T, PH, RT, KB> S;
+    constexpr int NT = S::NT, WK = S::WK, NB = S::NB, U = S::U, CW = S::CW, SPT = S::SPT;
+    constexpr int NRED = WK > 1 ? S::WR * (WK / 2) * NB * 256 : 1;
+    typedef typename std::conditional<PH == PH_F, Raw3<T>, cx<T>>::type Staged;
+    struct Pre {       // forward finish of a tile: gathered rows and parent rows
+        Raw2<T> g[PH == PH_F ? NB : 1][4];
+        int32_t prow[4];
+    };
+    struct Data {      // one step's loads
+        cx<T> m[U];    // factor entries (raw; masks at use)
+        Staged v[SPT]; // staged inner operand (raw)
+        Pre pre;       // forward, last step of a tile: its finish data
+    };
+    struct Idx {       // backward: panel rows of one step's update columns
+        int32_t x[PH == PH_B ? SPT : 1];
+    };
+    __shared__ SolveTile tq[kMvpTiles];
+    __shared__ cx<T> vs[2][CW][KB];
+    __shared__ T red[NRED];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave / WK, wk = wave % WK;
+    const int li = lane & 15, lk = lane >> 4;
+    const int G = gridDim.x;
+    const int nmine = (count - (int)blockIdx.x + G - 1) / G;
+    {
+        const int4* src = reinterpret_cast<const int4*>(tiles + start);
+        int4* dst = reinterpret_cast<int4*>(tq);
+        for (int e = tid; e < nmine * 4; e += NT)
+            dst[e] = src[((int64_t)blockIdx.x + (int64_t)(e >> 2) * G) * 4 + (e & 3)];
+    }
+    LdsBarrier();
+    if (nmine <= 0) return;
+
+    struct Cur {
+        int i, c0;  // tile (of this workgroup) and first inner column of the step
+    };
+    auto ncolsOf = [&](int i) { return max(tq[i].col1 - tq[i].col0, 1); };
+    auto advance = [&](Cur c) {
+        if (c.i >= nmine) return c;  // past the end stays there
+        c.c0 += CW;
+        if (c.c0 >= ncolsOf(c.i)) {
+            ++c.i;
+            c.c0 = 0;
+        }
+        return c;
+    };
+    // inner column of step element c (clamped into the tile: every load
+    // below reads a valid address; masks are applied when the data is used)
+    auto colAt = [&](const SolveTile& t, Cur c, int cl) {
+        const int cb = t.col0 + c.c0;
+        return cb + max(min(cl, t.col1 - cb - 1), 0);
+    };
+    auto issueIdx = [&](Cur c, Idx& ix) {
+        if constexpr (PH == PH_B) {
+            const SolveTile& t = tq[c.i];
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                const int col = colAt(t, c, (tid + NT * q) / KB);
+                ix.x[q] = structs[t.nd.ustart + max(col - t.nd.s, 0)];
+            }
+        }
+    };
+    auto issuePre = [&](int i, Pre& pr) {
+        if constexpr (PH == PH_F) {
+            const SolveTile& t = tq[i];
+            const NodeDev nd = t.nd;
+            const int fs = nd.s + nd.u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = min(t.row0 + 16 * wr + Mfma<T>::row(lk, q), fs - 1);
+                pr.prow[q] = gs.relinds[nd.ustart + max(row - nd.s, 0)];
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = min(t.row0 + 16 * wr + Mfma<T>::row(lk, q), fs - 1);
+                    const int64_t o = (nd.w + row) * k + min(8 * b + li, k - 1);
+                    pr.g[b][q] = Raw2<T>{gs.G[o], gs.G[gs.half + o]};
+                }
+        }
+    };
+    auto issueData = [&](Cur c, Data& d, Idx& ix) {
+        const SolveTile& t = tq[c.i];
+        const NodeDev nd = t.nd;
+        const int s = nd.s, fs = nd.s + nd.u;
+        const int arow = t.row0 + min(16 * wr + li, t.nrows - 1);
+        const cx<T>* Lf = L + nd.l;
+        const cx<T>* Lt = LT + nd.l;
+        if constexpr (PH == PH_B) {
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) ix.x[q] = Opaque(ix.x[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            const int e = tid + NT * q;
+            const int col = colAt(t, c, e / KB), rr = min(e % KB, k - 1);
+            if constexpr (PH == PH_F) {
+                const int64_t g = (nd.w + col) * k + rr;
+                d.v[q] = Raw3<T>{gs.G[g], gs.G[gs.half + g], Xp[(int64_t)(nd.off + col) * k + rr]};
+            } else {
+                const int64_t row = col < s ? nd.w + col : (int64_t)ix.x[q];
+                d.v[q] = (col < s ? W : Xp)[row * k + rr];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int col = colAt(t, c, 4 * (wk + WK * u) + lk);
+            d.m[u] = PH == PH_F ? Lf[arow + (int64_t)col * fs] : Lt[arow + (int64_t)col * s];
+        }
+    };
+    acc_t acc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = acc_t{0, 0, 0, 0};
+    auto finishTile = [&](int i, const Pre& pr) {
+        if (WK > 1) {  // waves that share rows sum their partial tiles by a fixed tree
+            for (int h = WK / 2; h >= 1; h >>= 1) {
+                LdsBarrier();
+                if (wk >= h && wk < 2 * h) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            red[(((wr * h + wk - h) * NB + b) * 64 + lane) * 4 + q] = acc[b][q];
+                }
+                LdsBarrier();
+                if (wk < h) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            acc[b][q] += red[(((wr * h + wk) * NB + b) * 64 + lane) * 4 + q];
+                }
+            }
+        }
+        if (wk == 0) {
+            const SolveTile& t = tq[i];
+            const NodeDev nd = t.nd;
+            const int nrows = t.nrows, row0 = t.row0, slot = t.slot;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const T val = acc[b][q];
+                    const T other = __shfl_xor(val, 8, 64);
+                    const int rl = 16 * wr + Mfma<T>::row(lk, q), rr = 8 * b + li;
+                    if (li < 8 && rl < nrows && rr < k) {
+                        const cx<T> a{val, other};
+                        if (slot >= 0)
+                            P[((int64_t)slot * 64 + rl) * k + rr] = a;
+                        else if constexpr (PH == PH_F)
+                            Finish<T, PH>(nd, gs, Xp, W, row0 + rl, rr, k, a,
+                                          pr.g[b][q].a + pr.g[b][q].b, pr.prow[q]);
+                        else
+                            Finish<T, PH>(nd, gs, Xp, W, row0 + rl, rr, k, a, czero<T>(), 0);
+                    }
+                }
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = acc_t{0, 0, 0, 0};
+    };
+    auto consume = [&](Cur c, const Data& d, cx<T> (*vb)[KB]) {
+        const SolveTile& t = tq[c.i];
+        const int cb = t.col0 + c.c0, nc = t.col1 - cb;
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            const int e = tid + NT * q, cl = e / KB, r = e % KB;
+            cx<T> v;
+            if constexpr (PH == PH_F)
+                v = d.v[q].a + d.v[q].b + d.v[q].c;
+            else
+                v = d.v[q];
+            vb[cl][r] = (cl < nc && r < k) ? v : czero<T>();
+        }
+        LdsBarrier();
+        const int rl = 16 * wr + li, arow = t.row0 + rl;
+        const bool rowOk = rl < t.nrows;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int cl = 4 * (wk + WK * u) + lk, col = cb + cl;
+            const bool ok = rowOk && cl < nc && (PH == PH_F ? col <= arow : col >= arow);
+            const cx<T> a = ok ? d.m[u] : czero<T>();
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const cx<T> v = vb[cl][8 * b + (li & 7)];
+                const T b1 = li < 8 ? v.r : v.i;
+                const T b2 = li < 8 ? -v.i : v.r;
+                acc[b] = Mfma<T>::step(a.r, b1, acc[b]);
+                acc[b] = Mfma<T>::step(a.i, b2, acc[b]);
+            }
+        }
+        if (c.c0 + CW >= ncolsOf(c.i)) finishTile(c.i, d.pre);
+        if constexpr (PH == PH_F) {  // the finish data was issued one step ago
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    Sink(d.pre.g[b][q].a.r), Sink(d.pre.g[b][q].a.i);
+                    Sink(d.pre.g[b][q].b.r), Sink(d.pre.g[b][q].b.i);
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Sink(d.pre.prow[q]);
+        }
+    };
+
+    Data d0, d1;
+    Idx i0, i1;
+    // steps n, n + 1, n + 2: data of n sits in d[n % 2], its indices were in
+    // i[n % 2]; a cursor is past the end when its tile index reaches nmine
+    Cur cn{0, 0};
+    Cur cn1 = advance(cn);
+    Cur cn2 = advance(cn1);
+    // The index and data loads are issued unconditionally (a cursor past the
+    // end reloads the current step, unused), so the wait counts before a
+    // step's use never have to allow for a path without the loads issued
+    // after it.  The finish data of a tile travels with its last step in the
+    // same register set; it is issued after the multiply of the previous
+    // step, one step before it is used.
+    auto lastStep = [&](Cur c) { return c.i < nmine && c.c0 + CW >= ncolsOf(c.i); };
+    issueIdx(cn, i0);
+    issueIdx(cn1.i < nmine ? cn1 : cn, i1);
+    issueData(cn, d0, i0);
+    if (PH == PH_F && lastStep(cn)) issuePre(cn.i, d0.pre);
+    auto half = [&](Idx& iN, Idx& iN1, Data& dN, Data& dN1, cx<T> (*vb)[KB]) {
+        issueIdx(cn2.i < nmine ? cn2 : cn, iN);
+        issueData(cn1.i < nmine ? cn1 : cn, dN1, iN1);
+        consume(cn, dN, vb);
+        if (PH == PH_F && lastStep(cn1)) issuePre(cn1.i, dN1.pre);
+        cn = cn1;
+        cn1 = cn2;
+        cn2 = advance(cn2);
+    };
+    // On the way out, every load still in flight (the reloads past the end)
+    // is waited for: the compiler merges the two exits with the path back
+    // to the loop head, and a load pending on any of them would otherwise
+    // be drained at the head of every iteration.
+    auto drain = [&](const Data& d, const Idx& ix) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) Sink(d.m[u].r), Sink(d.m[u].i);
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            if constexpr (PH == PH_F)
+                Sink(d.v[q].a.r), Sink(d.v[q].a.i), Sink(d.v[q].b.r), Sink(d.v[q].b.i),
+                    Sink(d.v[q].c.r), Sink(d.v[q].c.i);
+            else
+                Sink(d.v[q].r), Sink(d.v[q].i), Sink(ix.x[q]);
+        }
+    };
+    for (;;) {
+        half(i0, i1, d0, d1, vs[0]);
+        if (cn.i >= nmine) {
+            drain(d1, i0);
+            break;
+        }
+        half(i1, i0, d1, d0, vs[1]);
+        if (cn.i >= nmine) {
+            drain(d0, i1);
+            break;
         }
     }
 }
@@ -1590,6 +1908,39 @@ void LaunchInvDiag(const ClassDev& c, int nb, hipStream_t s)
                        (cx<T>*)c.L, c.fstore);
 }
 
+// Workgroups of a persistent solve launch: as many as fit on the device at
+// once, more only when a workgroup would otherwise get over kMvpTiles tiles.
+template <typename T, int PH, int RT, int KB>
+int MvpGrid(int cus, int count, int cap0 = 0)
+{
+    static const int perCU = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_mvp<T, PH, RT, KB>, 256, 0) !=
+                hipSuccess ||
+            b < 1)
+            b = 1;
+        return b;
+    }();
+    const int fit = cap0 > 0 ? cap0 : perCU * std::max(cus, 1);
+    const int cap = std::max(fit, (count + kMvpTiles - 1) / kMvpTiles);
+    return std::max(1, std::min(count, cap));
+}
+
+void PrimeSolveOccupancy()
+{
+    auto prime = [](auto t) {
+        typedef decltype(t) T;
+        (void)MvpGrid<T, PH_F, 64, 8>(1, 1), (void)MvpGrid<T, PH_F, 32, 8>(1, 1);
+        (void)MvpGrid<T, PH_F, 16, 8>(1, 1), (void)MvpGrid<T, PH_F, 64, 16>(1, 1);
+        (void)MvpGrid<T, PH_F, 32, 16>(1, 1), (void)MvpGrid<T, PH_F, 16, 16>(1, 1);
+        (void)MvpGrid<T, PH_B, 64, 8>(1, 1), (void)MvpGrid<T, PH_B, 32, 8>(1, 1);
+        (void)MvpGrid<T, PH_B, 16, 8>(1, 1), (void)MvpGrid<T, PH_B, 64, 16>(1, 1);
+        (void)MvpGrid<T, PH_B, 32, 16>(1, 1), (void)MvpGrid<T, PH_B, 16, 16>(1, 1);
+    };
+    prime(float());
+    prime(double());
+}
+
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, void* Pv,
                       void* Gv, int k, hipStream_t s, MvProfile* prof, int onlyLevel,
@@ -1615,6 +1966,25 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
             else
                 hipLaunchKernelGGL((k_mmw<T, PH, 16>), dim3(r.count), dim3(256), 0, s, c.mv,
                                    r.start, c.structs, gs, L, LT, Xp, W, P, k);
+        } else if (k >= 8 && k <= 16 && r.persist && r.waves == 4) {
+            auto launchP = [&](auto rt, auto kb) {
+                constexpr int RT = decltype(rt)::value, KB = decltype(kb)::value;
+                const int G = MvpGrid<T, PH, RT, KB>(c.cus, r.count, c.mvpGrid);
+                hipLaunchKernelGGL((k_mvp<T, PH, RT, KB>), dim3(G), dim3(256), 0, s, c.mv,
+                                   r.start, r.count, c.structs, gs, L, LT, Xp, W, P, k);
+            };
+            auto byRowsP = [&](auto kb) {
+                if (r.rows == 64)
+                    launchP(std::integral_constant<int, 64>(), kb);
+                else if (r.rows == 32)
+                    launchP(std::integral_constant<int, 32>(), kb);
+                else
+                    launchP(std::integral_constant<int, 16>(), kb);
+            };
+            if (k <= 8)
+                byRowsP(std::integral_constant<int, 8>());
+            else
+                byRowsP(std::integral_constant<int, 16>());
         } else if (k >= 8) {
             auto launch = [&](auto rt, auto kb) {
                 constexpr int RT = decltype(rt)::value, KB = decltype(kb)::value;

@@ -77,11 +77,16 @@ struct ClassDev {
     void* U = nullptr;              // npanels * ustore
     void* T = nullptr;              // npanels * tstore
     void* LT = nullptr;             // npanels * fstore: per front L^T (s x (s+u), ld s)
+    int cus = 0;                    // compute units of the device (persistent grids)
+    int mvpGrid = 0;                // persistent solve launches: workgroup cap (0: none)
 };
 
 
 // ---- launch wrappers (kernels.hip) ----
 void LaunchFillZero(void* p, int64_t bytes, hipStream_t s);
+// Occupancy of the persistent solve kernels, queried once (outside any
+// stream capture).
+void PrimeSolveOccupancy();
 
 template <typename T>
 void LaunchScatterOrig(const ClassDev& c, const StencilArgs& st, const double* vel,

@@ -60,6 +60,7 @@ struct psp_hip_ctx {
     c128 *partC = nullptr, *Hcol = nullptr, *ydev = nullptr;
     // captured sweeps, keyed on (block pointer, k)
     bool useGraph = true;
+    int mvpGrid = 0;  // persistent solve launches: workgroup cap (0: none)
     struct SweepGraph {
         void* B = nullptr;
         int k = 0;
@@ -255,10 +256,16 @@ int EnsureDevice(psp_hip_ctx* c)
     c->vel = (double*)vp;
 
     const size_t eb = FrontBytes(c);
+    int dev = 0, cus = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PrimeSolveOccupancy();
     for (int cl = 0; cl < 3; ++cl) {
         if (!c->plan.present[cl]) continue;
         const ClassPlan& P = c->plan.cls[cl];
         ClassDev& D = c->dev[cl];
+        D.cus = cus;
+        D.mvpGrid = c->mvpGrid;
         D.n = P.n;
         D.vsize = P.vsize;
         D.nnodes = (int)P.nodes.size();
@@ -882,9 +889,12 @@ int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
                 if (k >= 8 && k < 32)  // 16-wave workgroups exist on the matrix-core path
                     for (int rows : {32, 16})
                         for (int split : {1, 2}) cands.push_back(SolveShape{rows, split, 16});
+                if (k >= 8 && k <= 16)  // persistent pipelined workgroups
+                    for (int rows : {64, 32, 16})
+                        for (int split : {1, 2, 4}) cands.push_back(SolveShape{rows, split, 4, 1});
                 for (const SolveShape& cand : cands) {
                     if (rc || (cand.rows == cur.rows && cand.split == cur.split &&
-                               cand.waves == cur.waves))
+                               cand.waves == cur.waves && cand.persist == cur.persist))
                         continue;
                     double t = 0;
                     if ((rc = timeShape(d, ph, cand, &t))) break;
@@ -968,10 +978,20 @@ int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
         c->useGraph = value != 0;
         return PSP_HIP_OK;
     }
+    if (option == PSP_HIP_OPT_SOLVE_PERSIST_GRID) {
+        if (value < 0) return Fail(c, PSP_HIP_ERR_INVALID, "persistent grid cap must be >= 0");
+        if (c->devReady) (void)hipStreamSynchronize(c->stream);
+        FreeGraphs(c);  // captured launches keep their grid sizes
+        c->mvpGrid = value;
+        for (auto& D : c->dev) D.mvpGrid = value;
+        return PSP_HIP_OK;
+    }
     if (option == PSP_HIP_OPT_SOLVE_MIN_TILES || option == PSP_HIP_OPT_SOLVE_MAX_SPLIT ||
-        option == PSP_HIP_OPT_SOLVE_WIDE_WG) {
+        option == PSP_HIP_OPT_SOLVE_WIDE_WG || option == PSP_HIP_OPT_SOLVE_PERSIST) {
         if (option == PSP_HIP_OPT_SOLVE_WIDE_WG) {
             c->plan.wideWg = value != 0;
+        } else if (option == PSP_HIP_OPT_SOLVE_PERSIST) {
+            c->plan.persist = value != 0;
         } else if (option == PSP_HIP_OPT_SOLVE_MIN_TILES) {
             if (value < 1 || value > (1 << 20))
                 return Fail(c, PSP_HIP_ERR_INVALID, "solve min tiles out of range");
@@ -982,8 +1002,9 @@ int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
             c->plan.maxSplit = value;
         }
         for (int cl = 0; cl < 3; ++cl)
-            if (c->plan.present[cl]) DefaultSolveShapes(c->plan.cls[cl], c->plan.minTiles,
-                                                        c->plan.maxSplit, c->plan.wideWg);
+            if (c->plan.present[cl])
+                DefaultSolveShapes(c->plan.cls[cl], c->plan.minTiles, c->plan.maxSplit,
+                                   c->plan.wideWg, c->plan.persist);
         return RebuildSolveTiles(c);
     }
     return Fail(c, PSP_HIP_ERR_INVALID, "unknown option");

@@ -529,7 +529,7 @@ void Schedule(const Plan& plan, ClassPlan& C)
                     C.ginv[(size_t)2 * (n.woff + r) + k] = (int32_t)(ch.woff + ch.size + a);
                 }
             }
-        DefaultSolveShapes(C, plan.minTiles, plan.maxSplit, plan.wideWg);
+        DefaultSolveShapes(C, plan.minTiles, plan.maxSplit, plan.wideWg, plan.persist);
         BuildSolveTiles(C);
         // Solve layout: LT[:, :s] = Linv^T D^-1 (32x32 blocks of L_TL),
         // LT[:, s:] = (-L_BL Linv)^T by one batched GEMM, then L_BL and the
@@ -609,19 +609,19 @@ int64_t RowBlocks(const ClassPlan& C, int d, int ph, int rows)
 
 }  // namespace
 
-void DefaultSolveShapes(ClassPlan& C, int minTiles, int maxSplit, bool wideWg)
+void DefaultSolveShapes(ClassPlan& C, int minTiles, int maxSplit, bool wideWg, bool persist)
 {
     C.shapes.assign(C.maxDepth + 1, {});
     for (int d = 0; d <= C.maxDepth; ++d)
         for (int ph = 0; ph < 2; ++ph) {
-            SolveShape sh{16, 1, 4};
+            SolveShape sh{16, 1, 4, persist ? 1 : 0};
             for (int cand : {64, 32, 16}) {
                 sh.rows = cand;
                 if (RowBlocks(C, d, ph, cand) >= minTiles) break;
             }
             const int64_t t = RowBlocks(C, d, ph, sh.rows);
             if (t > 0 && t < minTiles) {
-                if (wideWg)
+                if (wideWg && !persist)
                     sh.waves = 16;
                 else
                     sh.split = (int)std::min<int64_t>(maxSplit, CeilDiv(minTiles, t));
@@ -646,6 +646,7 @@ void BuildSolveTiles(ClassPlan& C)
         const SolveShape sh = C.shapes[d][ph];
         rg.rows = sh.rows;
         rg.waves = sh.waves;
+        rg.persist = sh.persist;
         rg.start = (int)C.mv.size();
         rg.rstart = (int)C.mvred.size();
         int slots = 0;

@@ -120,6 +120,7 @@ struct SolveShape {
     int split = 1;  // inner slices per row block (partial sums + reduce)
     int waves = 4;  // waves per workgroup (4, or 16 to spread the inner range
                     // of a tile over more waves instead of slices), 8 <= k < 32
+    int persist = 0;  // 1: persistent pipelined workgroups (8 <= k <= 16, 4 waves)
 };
 
 // 32x32 block (rows i0.., columns j0..) of a per-front copy between L and
@@ -200,6 +201,7 @@ struct ClassPlan {
         int start = 0, count = 0;    // tiles
         int rows = 64;               // output rows per tile (64, 32 or 16)
         int waves = 4;               // waves per workgroup of the matrix-core path
+        int persist = 0;             // persistent pipelined workgroups
         int rstart = 0, rcount = 0;  // reduce items
     };
     struct SolveLevel {
@@ -226,6 +228,7 @@ struct Plan {
     int minTiles = 256;    // solve tiles wanted per launch (default shapes)
     int maxSplit = 8;      // inner slices per row block at most (default shapes)
     bool wideWg = false;   // default shapes: 16-wave workgroups instead of slices
+    bool persist = true;   // default shapes: persistent pipelined workgroups
 };
 
 // Builds everything; returns PSP_HIP_OK or an error code with `msg` set.
@@ -234,7 +237,8 @@ int BuildPlan(const psp_hip_desc& desc, Plan& plan, std::string& msg);
 // Solve tile shapes: the default per (level, phase) is the largest row
 // block of 64, 32, 16 rows that gives minTiles tiles, and otherwise inner
 // slices (at most maxSplit) to reach minTiles.
-void DefaultSolveShapes(ClassPlan& C, int minTiles, int maxSplit, bool wideWg = false);
+void DefaultSolveShapes(ClassPlan& C, int minTiles, int maxSplit, bool wideWg = false,
+                        bool persist = false);
 // (Re)builds the solve tiles, reduce items and their accounting from
 // C.shapes.
 void BuildSolveTiles(ClassPlan& C);

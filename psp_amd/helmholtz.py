@@ -205,13 +205,17 @@ class DistHelmholtz:
             _raise(st, self._h, "Autotune")
         return b.value, a.value
 
-    def set_solve_tiling(self, min_tiles=None, max_split=None, wide_wg=None):
+    def set_solve_tiling(self, min_tiles=None, max_split=None, wide_wg=None, persist=None,
+                         persist_grid=None):
         """Default front-solve tile shapes: workgroups wanted per launch, the
-        most inner slices per row block, and whether levels with few row
-        blocks get 16-wave workgroups instead of slices (rebuilds the solve
-        tiles)."""
+        most inner slices per row block, whether levels with few row blocks
+        get 16-wave workgroups instead of slices, and whether the solves run
+        on persistent pipelined workgroups (8 <= k <= 16); rebuilds the solve
+        tiles.  persist_grid caps the workgroups of a persistent launch
+        (0: as many as fit on the device)."""
         for opt, v in ((_lib.OPT_SOLVE_MIN_TILES, min_tiles), (_lib.OPT_SOLVE_MAX_SPLIT, max_split),
-                       (_lib.OPT_SOLVE_WIDE_WG, wide_wg)):
+                       (_lib.OPT_SOLVE_WIDE_WG, wide_wg), (_lib.OPT_SOLVE_PERSIST, persist),
+                       (_lib.OPT_SOLVE_PERSIST_GRID, persist_grid)):
             if v is None:
                 continue
             st = _lib.lib().psp_hip_set_option(self._ctx(), opt, int(v))

@@ -282,8 +282,67 @@ def test_wide_workgroup_shapes_match_oracle(torch_cuda, k):
     disc, vel = _problem(16, 16, 20)
     o = _oracle(disc, vel, 5)
     h = _helm(disc, vel, cutoff=5)
-    h.set_solve_tiling(min_tiles=100000, wide_wg=True)  # every level counts as "few tiles"
+    # every level counts as "few tiles"; one workgroup per tile
+    h.set_solve_tiling(min_tiles=100000, wide_wg=True, persist=False)
     rng = np.random.default_rng(90 + k)
+    l2n = o.local_to_natural()
+    X = rand_block(rng, o.N, k)
+    xd = torch.from_numpy(X).cuda()
+    h.Precondition(xd)
+    torch.cuda.synchronize()
+    assert _rel(xd.cpu().numpy()[l2n], o.precondition(X[l2n])) < 1e-10
+
+
+@pytest.mark.parametrize("k", [8, 13, 16])
+@pytest.mark.parametrize("grid,min_tiles", [(0, None), (2, None), (3, 100000)])
+def test_persistent_solves_match_oracle(torch_cuda, k, grid, min_tiles):
+    """Persistent pipelined workgroups: with a device-filling grid, with two
+    or three workgroups (each works through many tiles, so the step pipeline
+    crosses tiles and nodes), and with inner slices (min_tiles forces every
+    level to split): panel solves and the sweep match the oracle."""
+    torch = torch_cuda
+    disc, vel = _problem(16, 16, 20)
+    o = _oracle(disc, vel, 5)
+    h = _helm(disc, vel, cutoff=5)
+    h.set_solve_tiling(min_tiles=min_tiles, persist=True, persist_grid=grid)
+    rng = np.random.default_rng(120 + k + grid)
+    l2n = o.local_to_natural()
+    X = rand_block(rng, o.N, k)
+    xd = torch.from_numpy(X).cuda()
+    h.Precondition(xd)
+    torch.cuda.synchronize()
+    assert _rel(xd.cpu().numpy()[l2n], o.precondition(X[l2n])) < 1e-10
+
+
+@pytest.mark.parametrize("k", [8, 16])
+def test_persistent_fp32_sweep_close_to_fp64(torch_cuda, k):
+    """FP32 fronts on the persistent path agree with the FP64 sweep to FP32
+    accuracy, column by column."""
+    torch = torch_cuda
+    disc, vel = _problem(16, 14, 26)
+    h64 = _helm(disc, vel, "fp64")
+    h32 = _helm(disc, vel, "fp32")
+    h32.set_solve_tiling(persist=True, persist_grid=4)
+    rng = np.random.default_rng(60 + k)
+    X = torch.from_numpy(rand_block(rng, disc.nx * disc.ny * disc.nz, k)).cuda()
+    y64, y32 = X.clone(), X.clone()
+    h64.Precondition(y64)
+    h32.Precondition(y32)
+    torch.cuda.synchronize()
+    err = (torch.linalg.norm(y32 - y64, dim=0) / torch.linalg.norm(y64, dim=0)).max().item()
+    assert err < 1e-4, err
+
+
+@pytest.mark.parametrize("k", [8, 13, 16])
+def test_tile_per_workgroup_solves_match_oracle(torch_cuda, k):
+    """One workgroup per tile (the non-persistent matrix-core kernels), with
+    default shapes: the sweep matches the oracle."""
+    torch = torch_cuda
+    disc, vel = _problem(16, 16, 20)
+    o = _oracle(disc, vel, 5)
+    h = _helm(disc, vel, cutoff=5)
+    h.set_solve_tiling(persist=False)
+    rng = np.random.default_rng(150 + k)
     l2n = o.local_to_natural()
     X = rand_block(rng, o.N, k)
     xd = torch.from_numpy(X).cuda()
