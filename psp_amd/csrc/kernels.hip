@@ -69,6 +69,12 @@ __device__ __forceinline__ cx<T> cdivr(cx<T> a, T s) { return {a.r / s, a.i / s}
 template <typename To, typename Ti>
 __device__ __forceinline__ cx<To> cvt(cx<Ti> a) { return {To(a.r), To(a.i)}; }
 
+// Two reals of one type (a pre-swizzled LDS operand pair).
+template <typename T>
+struct T2 {
+    T x, y;
+};
+
 template <typename T>
 __device__ __forceinline__ cx<T> czero() { return {T(0), T(0)}; }
 
@@ -861,6 +867,16 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
 // order within a tile (4-column steps in increasing order, the waves of a
 // row block summed by the same fixed tree) and the finish are those of
 // k_mvm; only the schedule differs.  No workgroup waits for another.
+//
+// The loop is written for few vector instructions per load: a tile's
+// fields are made wave-uniform (so its base addresses are scalar), every
+// lane's offsets within a step are fixed for the launch, and factor entries
+// are used unmasked.  That relies on the solve storage: the triangles a
+// phase does not use are zero (k_zero_triangles), the factor and vector
+// buffers end in kSolvePad padding columns / rows of zeros, so the entries
+// a partial step reads past its tile are finite and meet a zero in the
+// staged vector (the staged vector is masked), and rows past a partial
+// tile read the tile's last row and give outputs that are not stored.
 constexpr int kMvpTiles = 32;  // tiles per workgroup at most (host sizes the grid)
 #ifndef PSP_MVP_U
 #define PSP_MVP_U 8  // factor loads per lane per step, 64- and 32-row tiles
@@ -876,7 +892,16 @@ struct MvpShape {
     static constexpr int CW = 4 * WK * U;        // inner columns per step
     static constexpr int SPT = CW * KB / NT;     // staged elements per thread per step
     static_assert(CW * KB == SPT * NT && SPT >= 1, "step shape");
+    static_assert(CW <= kSolvePad, "a step reads at most kSolvePad columns past its tile");
 };
+
+__device__ __forceinline__ int Uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t Uni(int64_t v)
+{
+    const uint64_t u = (uint64_t)v;
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
+}
 
 template <typename T, int PH, int RT, int KB>
 __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, int count,
@@ -894,15 +919,21 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
         int32_t prow[4];
     };
     struct Data {      // one step's loads
-        cx<T> m[U];    // factor entries (raw; masks at use)
+        cx<T> m[U];    // factor entries
         Staged v[SPT]; // staged inner operand (raw)
         Pre pre;       // forward, last step of a tile: its finish data
     };
     struct Idx {       // backward: panel rows of one step's update columns
         int32_t x[PH == PH_B ? SPT : 1];
     };
+    struct Tile {      // wave-uniform fields of a tile
+        int64_t l, w;
+        int off, s, u, ustart, pw, cslot, row0, nrows, col0, col1, slot;
+    };
     __shared__ SolveTile tq[kMvpTiles];
-    __shared__ cx<T> vs[2][CW][KB];
+    // staged vector in the two forms the B operand takes: lanes l % 16 < 8
+    // read (re, -im), the others (im, re)
+    __shared__ T2<T> vs[2][2][CW][KB];
     __shared__ T red[NRED];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave / WK, wk = wave % WK;
@@ -917,11 +948,35 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
     }
     LdsBarrier();
     if (nmine <= 0) return;
+    auto tile = [&](int i) {
+        const SolveTile& t = tq[i];
+        Tile r;
+        r.l = Uni(t.nd.l);
+        r.w = Uni(t.nd.w);
+        r.off = Uni(t.nd.off);
+        r.s = Uni(t.nd.s);
+        r.u = Uni(t.nd.u);
+        r.ustart = Uni(t.nd.ustart);
+        r.pw = Uni(t.nd.pw);
+        r.cslot = Uni(t.nd.cslot);
+        r.row0 = Uni(t.row0);
+        r.nrows = Uni(t.nrows);
+        r.col0 = Uni(t.col0);
+        r.col1 = Uni(t.col1);
+        r.slot = Uni(t.slot);
+        return r;
+    };
+    // lane constants of the launch
+    const int lrow = 16 * wr + li;        // factor row within the tile
+    const int lcol = 4 * wk + lk;         // factor column within a 4 WK-column group
+    int sc[SPT], sr[SPT];                 // staged element: column within the step, rhs
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) sc[q] = (tid + NT * q) / KB, sr[q] = (tid + NT * q) % KB;
 
     struct Cur {
         int i, c0;  // tile (of this workgroup) and first inner column of the step
     };
-    auto ncolsOf = [&](int i) { return max(tq[i].col1 - tq[i].col0, 1); };
+    auto ncolsOf = [&](int i) { return max(Uni(tq[i].col1) - Uni(tq[i].col0), 1); };
     auto advance = [&](Cur c) {
         if (c.i >= nmine) return c;  // past the end stays there
         c.c0 += CW;
@@ -931,69 +986,64 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
         }
         return c;
     };
-    // inner column of step element c (clamped into the tile: every load
-    // below reads a valid address; masks are applied when the data is used)
-    auto colAt = [&](const SolveTile& t, Cur c, int cl) {
-        const int cb = t.col0 + c.c0;
-        return cb + max(min(cl, t.col1 - cb - 1), 0);
-    };
     auto issueIdx = [&](Cur c, Idx& ix) {
         if constexpr (PH == PH_B) {
-            const SolveTile& t = tq[c.i];
+            const int ustart = Uni(tq[c.i].nd.ustart), s = Uni(tq[c.i].nd.s);
+            const int cb = Uni(tq[c.i].col0) + c.c0;
+            const int32_t* sb = structs + ustart;  // read up to kSolvePad past the list
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) ix.x[q] = sb[max(cb + sc[q] - s, 0)];
+        }
+    };
+    auto issueData = [&](Cur c, Data& d, Idx& ix) {
+        const Tile t = tile(c.i);
+        const int cb = t.col0 + c.c0;
+        const int fa = PH == PH_F ? t.s + t.u : t.s;  // column stride of the factor block
+        const cx<T>* Ab = (PH == PH_F ? L : LT) + t.l + t.row0 + (int64_t)cb * fa;
+        const uint32_t la = (uint32_t)(min(lrow, t.nrows - 1) + lcol * fa);
+        const int64_t ustep = (int64_t)(4 * WK) * fa;
+        if constexpr (PH == PH_B) {
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) ix.x[q] = Opaque(ix.x[q]);
+        }
+        if constexpr (PH == PH_F) {
+            const cx<T>* g0 = gs.G + (t.w + cb) * k;
+            const cx<T>* g1 = g0 + gs.half;
+            const cx<T>* xb = Xp + (int64_t)(t.off + cb) * k;
 #pragma unroll
             for (int q = 0; q < SPT; ++q) {
-                const int col = colAt(t, c, (tid + NT * q) / KB);
-                ix.x[q] = structs[t.nd.ustart + max(col - t.nd.s, 0)];
+                const uint32_t o = (uint32_t)(sc[q] * k + sr[q]);
+                d.v[q] = Raw3<T>{g0[o], g1[o], xb[o]};
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                const int col = cb + sc[q];
+                const int64_t row = col < t.s ? t.w + col : (int64_t)ix.x[q];
+                d.v[q] = (col < t.s ? W : Xp)[row * k + sr[q]];
             }
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) d.m[u] = (Ab + u * ustep)[la];
     };
     auto issuePre = [&](int i, Pre& pr) {
         if constexpr (PH == PH_F) {
-            const SolveTile& t = tq[i];
-            const NodeDev nd = t.nd;
-            const int fs = nd.s + nd.u;
+            const Tile t = tile(i);
+            const int fs = t.s + t.u;
+            const int32_t* rb = gs.relinds + t.ustart;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int row = min(t.row0 + 16 * wr + Mfma<T>::row(lk, q), fs - 1);
-                pr.prow[q] = gs.relinds[nd.ustart + max(row - nd.s, 0)];
+                pr.prow[q] = rb[max(row - t.s, 0)];
             }
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int row = min(t.row0 + 16 * wr + Mfma<T>::row(lk, q), fs - 1);
-                    const int64_t o = (nd.w + row) * k + min(8 * b + li, k - 1);
+                    const int64_t o = (t.w + row) * k + min(8 * b + li, k - 1);
                     pr.g[b][q] = Raw2<T>{gs.G[o], gs.G[gs.half + o]};
                 }
-        }
-    };
-    auto issueData = [&](Cur c, Data& d, Idx& ix) {
-        const SolveTile& t = tq[c.i];
-        const NodeDev nd = t.nd;
-        const int s = nd.s, fs = nd.s + nd.u;
-        const int arow = t.row0 + min(16 * wr + li, t.nrows - 1);
-        const cx<T>* Lf = L + nd.l;
-        const cx<T>* Lt = LT + nd.l;
-        if constexpr (PH == PH_B) {
-#pragma unroll
-            for (int q = 0; q < SPT; ++q) ix.x[q] = Opaque(ix.x[q]);
-        }
-#pragma unroll
-        for (int q = 0; q < SPT; ++q) {
-            const int e = tid + NT * q;
-            const int col = colAt(t, c, e / KB), rr = min(e % KB, k - 1);
-            if constexpr (PH == PH_F) {
-                const int64_t g = (nd.w + col) * k + rr;
-                d.v[q] = Raw3<T>{gs.G[g], gs.G[gs.half + g], Xp[(int64_t)(nd.off + col) * k + rr]};
-            } else {
-                const int64_t row = col < s ? nd.w + col : (int64_t)ix.x[q];
-                d.v[q] = (col < s ? W : Xp)[row * k + rr];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int col = colAt(t, c, 4 * (wk + WK * u) + lk);
-            d.m[u] = PH == PH_F ? Lf[arow + (int64_t)col * fs] : Lt[arow + (int64_t)col * s];
         }
     };
     acc_t acc[NB];
@@ -1021,9 +1071,10 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
             }
         }
         if (wk == 0) {
-            const SolveTile& t = tq[i];
-            const NodeDev nd = t.nd;
-            const int nrows = t.nrows, row0 = t.row0, slot = t.slot;
+            const Tile t = tile(i);
+            NodeDev nd;
+            nd.l = t.l, nd.w = t.w, nd.off = t.off, nd.s = t.s, nd.u = t.u;
+            nd.ustart = t.ustart, nd.pw = t.pw, nd.cslot = t.cslot;
 #pragma unroll
             for (int b = 0; b < NB; ++b)
 #pragma unroll
@@ -1031,49 +1082,47 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
                     const T val = acc[b][q];
                     const T other = __shfl_xor(val, 8, 64);
                     const int rl = 16 * wr + Mfma<T>::row(lk, q), rr = 8 * b + li;
-                    if (li < 8 && rl < nrows && rr < k) {
+                    if (li < 8 && rl < t.nrows && rr < k) {
                         const cx<T> a{val, other};
-                        if (slot >= 0)
-                            P[((int64_t)slot * 64 + rl) * k + rr] = a;
+                        if (t.slot >= 0)
+                            P[((int64_t)t.slot * 64 + rl) * k + rr] = a;
                         else if constexpr (PH == PH_F)
-                            Finish<T, PH>(nd, gs, Xp, W, row0 + rl, rr, k, a,
+                            Finish<T, PH>(nd, gs, Xp, W, t.row0 + rl, rr, k, a,
                                           pr.g[b][q].a + pr.g[b][q].b, pr.prow[q]);
                         else
-                            Finish<T, PH>(nd, gs, Xp, W, row0 + rl, rr, k, a, czero<T>(), 0);
+                            Finish<T, PH>(nd, gs, Xp, W, t.row0 + rl, rr, k, a, czero<T>(), 0);
                     }
                 }
         }
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[b] = acc_t{0, 0, 0, 0};
     };
-    auto consume = [&](Cur c, const Data& d, cx<T> (*vb)[KB]) {
-        const SolveTile& t = tq[c.i];
-        const int cb = t.col0 + c.c0, nc = t.col1 - cb;
+    // B operand source of this lane (fixed for the launch)
+    const int form = li < 8 ? 0 : 1, bj = li & 7;
+    auto consume = [&](Cur c, const Data& d, int buf) {
+        const int nc = Uni(tq[c.i].col1) - (Uni(tq[c.i].col0) + c.c0);
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
-            const int e = tid + NT * q, cl = e / KB, r = e % KB;
             cx<T> v;
             if constexpr (PH == PH_F)
                 v = d.v[q].a + d.v[q].b + d.v[q].c;
             else
                 v = d.v[q];
-            vb[cl][r] = (cl < nc && r < k) ? v : czero<T>();
+            const bool ok = sc[q] < nc && sr[q] < k;
+            const T re = ok ? v.r : T(0), im = ok ? v.i : T(0);
+            vs[buf][0][sc[q]][sr[q]] = T2<T>{re, -im};
+            vs[buf][1][sc[q]][sr[q]] = T2<T>{im, re};
         }
         LdsBarrier();
-        const int rl = 16 * wr + li, arow = t.row0 + rl;
-        const bool rowOk = rl < t.nrows;
+        const T2<T>(*vb)[KB] = vs[buf][form];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int cl = 4 * (wk + WK * u) + lk, col = cb + cl;
-            const bool ok = rowOk && cl < nc && (PH == PH_F ? col <= arow : col >= arow);
-            const cx<T> a = ok ? d.m[u] : czero<T>();
+            const int cl = lcol + 4 * WK * u;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                const cx<T> v = vb[cl][8 * b + (li & 7)];
-                const T b1 = li < 8 ? v.r : v.i;
-                const T b2 = li < 8 ? -v.i : v.r;
-                acc[b] = Mfma<T>::step(a.r, b1, acc[b]);
-                acc[b] = Mfma<T>::step(a.i, b2, acc[b]);
+                const T2<T> w = vb[cl][8 * b + bj];
+                acc[b] = Mfma<T>::step(d.m[u].r, w.x, acc[b]);
+                acc[b] = Mfma<T>::step(d.m[u].i, w.y, acc[b]);
             }
         }
         if (c.c0 + CW >= ncolsOf(c.i)) finishTile(c.i, d.pre);
@@ -1093,25 +1142,25 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
     Data d0, d1;
     Idx i0, i1;
     // steps n, n + 1, n + 2: data of n sits in d[n % 2], its indices were in
-    // i[n % 2]; a cursor is past the end when its tile index reaches nmine
-    Cur cn{0, 0};
-    Cur cn1 = advance(cn);
-    Cur cn2 = advance(cn1);
+    // i[n % 2]; a cursor is past the end when its tile index reaches nmine.
     // The index and data loads are issued unconditionally (a cursor past the
     // end reloads the current step, unused), so the wait counts before a
     // step's use never have to allow for a path without the loads issued
     // after it.  The finish data of a tile travels with its last step in the
     // same register set; it is issued after the multiply of the previous
     // step, one step before it is used.
+    Cur cn{0, 0};
+    Cur cn1 = advance(cn);
+    Cur cn2 = advance(cn1);
     auto lastStep = [&](Cur c) { return c.i < nmine && c.c0 + CW >= ncolsOf(c.i); };
     issueIdx(cn, i0);
     issueIdx(cn1.i < nmine ? cn1 : cn, i1);
     issueData(cn, d0, i0);
     if (PH == PH_F && lastStep(cn)) issuePre(cn.i, d0.pre);
-    auto half = [&](Idx& iN, Idx& iN1, Data& dN, Data& dN1, cx<T> (*vb)[KB]) {
+    auto half = [&](Idx& iN, Idx& iN1, Data& dN, Data& dN1, int buf) {
         issueIdx(cn2.i < nmine ? cn2 : cn, iN);
         issueData(cn1.i < nmine ? cn1 : cn, dN1, iN1);
-        consume(cn, dN, vb);
+        consume(cn, dN, buf);
         if (PH == PH_F && lastStep(cn1)) issuePre(cn1.i, dN1.pre);
         cn = cn1;
         cn1 = cn2;
@@ -1134,16 +1183,34 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
         }
     };
     for (;;) {
-        half(i0, i1, d0, d1, vs[0]);
+        half(i0, i1, d0, d1, 0);
         if (cn.i >= nmine) {
             drain(d1, i0);
             break;
         }
-        half(i1, i0, d1, d0, vs[1]);
+        half(i1, i0, d1, d0, 1);
         if (cn.i >= nmine) {
             drain(d0, i1);
             break;
         }
+    }
+}
+
+// Zero the triangle of each front's top block that the solves never use (F:
+// above the diagonal of L's top block; B: below the diagonal of LT's), so
+// the persistent solve path can read factor entries without masks.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_zero_triangles(const NodeDev* nodes, cx<T>* L,
+                                                              cx<T>* LT, int64_t fstore)
+{
+    const NodeDev nd = nodes[blockIdx.x];
+    const int s = nd.s, fs = nd.s + nd.u;
+    cx<T>* Lf = L + (int64_t)blockIdx.y * fstore + nd.l;
+    cx<T>* Lt = LT + (int64_t)blockIdx.y * fstore + nd.l;
+    for (int64_t e = threadIdx.x; e < (int64_t)s * s; e += kThreads) {
+        const int i = (int)(e % s), j = (int)(e / s);
+        if (j > i) Lf[i + (int64_t)j * fs] = czero<T>();
+        if (j < i) Lt[i + (int64_t)j * s] = czero<T>();
     }
 }
 
@@ -2058,6 +2125,14 @@ void LaunchTransposeScale(const ClassDev& c, hipStream_t s)
 }
 
 template <typename T>
+void LaunchZeroTriangles(const ClassDev& c, hipStream_t s)
+{
+    if (c.nnodes <= 0) return;
+    hipLaunchKernelGGL((k_zero_triangles<T>), dim3(c.nnodes, c.npanels), dim3(kThreads), 0, s,
+                       c.nodes, (cx<T>*)c.L, (cx<T>*)c.LT, c.fstore);
+}
+
+template <typename T>
 void LaunchFoldBack(const ClassDev& c, hipStream_t s)
 {
     if (c.nfold <= 0) return;
@@ -2173,6 +2248,7 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
                                       void*, int, hipStream_t, MvProfile*, int, int);             \
     template void LaunchTransposeScale<T>(const ClassDev&, hipStream_t);                          \
     template void LaunchFoldBack<T>(const ClassDev&, hipStream_t);                                \
+    template void LaunchZeroTriangles<T>(const ClassDev&, hipStream_t);                           \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
                                   int, hipStream_t);                                              \
     template void LaunchCopyOut<T>(const ClassDev&, c128*, const void*, int, int, int, int, int,  \

@@ -50,6 +50,12 @@ struct SolveTile {
 };
 static_assert(sizeof(SolveTile) == 64, "SolveTile is one 64-byte record");
 
+// Padding of the solve buffers: the persistent solve kernels read up to this
+// many columns (factor blocks) or rows (vectors, struct lists) past a tile,
+// so the factor arrays, the panel vector, the work vectors, the slot copies
+// and the struct list end in this much zeroed padding.
+constexpr int kSolvePad = 64;
+
 // Per-class device copy of the static plan.
 struct ClassDev {
     int n = 0, vsize = 0, nnodes = 0, npanels = 0;
@@ -121,6 +127,9 @@ template <typename T>
 void LaunchTransposeScale(const ClassDev& c, hipStream_t s);
 template <typename T>
 void LaunchFoldBack(const ClassDev& c, hipStream_t s);
+// Zeroes the unused triangles of the fronts' top blocks in L and LT.
+template <typename T>
+void LaunchZeroTriangles(const ClassDev& c, hipStream_t s);
 // Natural-order <-> padded panel vector.  mode 0: B = Xp, 1: B += Xp.
 template <typename T>
 void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int ny, int nz,

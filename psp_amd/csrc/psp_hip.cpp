@@ -174,10 +174,10 @@ size_t FrontBytes(const psp_hip_ctx* c) { return c->desc.precision == PSP_HIP_FR
 // One-time device setup: upload the static plan and allocate factor storage.
 // The struct list with one trailing entry, so that a branch-free kernel
 // may read structs[ustart] of a front without an update (the root).
-std::vector<int32_t> WithSentinel(const std::vector<int>& v)
+std::vector<int32_t> WithSentinel(const std::vector<int>& v, int pad = 1)
 {
     std::vector<int32_t> out(v.begin(), v.end());
-    out.push_back(0);
+    out.resize(out.size() + pad, 0);
     return out;
 }
 
@@ -303,7 +303,7 @@ int EnsureDevice(psp_hip_ctx* c)
             (st = Upload(c, &D.qrows, P.qrows)) || (st = Upload(c, &D.nat2q, nat2q)) ||
             (st = Upload(c, &D.nodeL, nL)) || (st = Upload(c, &D.nodeW, nW)) ||
             (st = Upload(c, &D.nodeS, nS)) || (st = Upload(c, &D.nodeU, nU)) ||
-            (st = Upload(c, &D.nodeUs, nUs)) || (st = Upload(c, &D.structs, WithSentinel(P.structs))) ||
+            (st = Upload(c, &D.nodeUs, nUs)) || (st = Upload(c, &D.structs, WithSentinel(P.structs, kSolvePad + 1))) ||
             (st = Upload(c, &D.relinds, WithSentinel(P.relinds))) || (st = Upload(c, &D.extadd, P.extadd)) ||
             (st = Upload(c, &D.ldl, P.ldl)) || (st = Upload(c, &D.gemms, P.gemms)) ||
             (st = Upload(c, &D.tiles, P.tiles)) || (st = Upload(c, &D.inv, P.inv)) ||
@@ -320,8 +320,14 @@ int EnsureDevice(psp_hip_ctx* c)
         D.nfold = (int)P.foldBack.size();
         std::vector<c128> s3c = ToC128(s3a);
         if ((st = Upload(c, &D.s3art, s3c))) return st;
-        if ((st = Alloc(c, &D.L, eb * (size_t)D.npanels * D.fstore, c->allocs))) return st;
-        if ((st = Alloc(c, &D.LT, eb * (size_t)D.npanels * D.fstore, c->allocs))) return st;
+        // factor arrays end in kSolvePad columns of the tallest front (zeros)
+        int64_t maxFs = 1;
+        for (const Node& nd : P.nodes) maxFs = std::max<int64_t>(maxFs, nd.size + nd.usize);
+        const size_t fb = eb * ((size_t)D.npanels * D.fstore + (size_t)kSolvePad * maxFs);
+        if ((st = Alloc(c, &D.L, fb, c->allocs))) return st;
+        if ((st = Alloc(c, &D.LT, fb, c->allocs))) return st;
+        HIPCHK(c, hipMemsetAsync(D.L, 0, fb, c->stream));
+        HIPCHK(c, hipMemsetAsync(D.LT, 0, fb, c->stream));
     }
     c->devReady = true;
     return PSP_HIP_OK;
@@ -377,15 +383,19 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
         }
     int st;
     void* p;
-    if ((st = Alloc(c, &c->Xp, eb * maxn * k, c->wsAllocs))) return st;
-    if ((st = Alloc(c, &c->W, eb * maxw * k, c->wsAllocs))) return st;
+    // vectors end in kSolvePad + 2 zeroed rows (reads past a tile's rows)
+    const size_t padRows = (size_t)kSolvePad + 2;
+    if ((st = Alloc(c, &c->Xp, eb * (maxn + padRows) * k, c->wsAllocs))) return st;
+    if ((st = Alloc(c, &c->W, eb * (maxw + padRows) * k, c->wsAllocs))) return st;
+    HIPCHK(c, hipMemsetAsync(c->Xp, 0, eb * (maxn + padRows) * k, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->W, 0, eb * (maxw + padRows) * k, c->stream));
     if ((st = Alloc(c, &c->P, eb * maxs * 64 * k, c->wsAllocs))) return st;
     for (int cl = 0; cl < 3; ++cl) {
         c->G[cl] = nullptr;
         if (!c->plan.present[cl]) continue;
         // Zero once: each class writes the same rows of its slot copies in
         // every solve, and the rows it never writes must read as zero.
-        const size_t gb = eb * 2 * (size_t)c->plan.cls[cl].wstore * k;
+        const size_t gb = eb * (2 * (size_t)c->plan.cls[cl].wstore + padRows) * k;
         if ((st = Alloc(c, &c->G[cl], gb, c->wsAllocs))) return st;
         HIPCHK(c, hipMemsetAsync(c->G[cl], 0, gb, c->stream));
     }
@@ -496,6 +506,7 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv, dou
         LaunchTransposeScale<T>(D, s);
         LaunchGemm<T>(D, P.fold.tileStart, P.fold.tileCount, s);
         LaunchFoldBack<T>(D, s);
+        LaunchZeroTriangles<T>(D, s);
         HIPCHK(c, hipEventRecord(e[3], s));
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventSynchronize(e[3]));
