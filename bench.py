@@ -30,10 +30,10 @@ def log(*a):
 
 
 def measured_traffic():
-    """HBM bytes per k_mv launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc)
-    recorded in the committed profile summary, or None."""
+    """HBM bytes per front-solve product launch (FETCH_SIZE + WRITE_SIZE,
+    rocprofv3 --pmc) recorded in the committed profile summary, or None."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_mvm_front_solves_bumps100_8rhs.md")
+    path = os.path.join(ROOT, "profiles", "k_mvp_k_mvm_front_solves_bumps100_8rhs.md")
     try:
         m = re.search(r"\*\*([0-9.]+) MB per launch\*\* of HBM traffic", open(path).read())
         return float(m.group(1)) * 1e6 if m else None
@@ -334,7 +334,7 @@ def main():
     # the sweep from a hipGraph, whose kernels cannot be bracketed singly).
     launches, ksec, kbytes, kflops = h32.ProfileSweep(Bsrc.clone())
     achieved = kbytes / ksec / 1e9
-    kname = "k_mvm" if k >= 8 and k < 32 else ("k_mmw" if k >= 32 else "k_mv")
+    kname = ("k_mvp/k_mvm" if k <= 16 else "k_mvm") if 8 <= k < 32 else ("k_mmw" if k >= 32 else "k_mv")
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                 "frac": achieved / 8000.0,
                 "traffic": measured_traffic() if (k == 8 and args.workload == "bumps100") else None,
