@@ -333,10 +333,11 @@ def test_persistent_fp32_sweep_close_to_fp64(torch_cuda, k):
     assert err < 1e-4, err
 
 
-@pytest.mark.parametrize("k", [8, 13, 16])
+@pytest.mark.parametrize("k", [8, 13, 16, 24])
 def test_tile_per_workgroup_solves_match_oracle(torch_cuda, k):
     """One workgroup per tile (the non-persistent matrix-core kernels), with
-    default shapes: the sweep matches the oracle."""
+    default shapes: the sweep matches the oracle (k = 24: two passes of 16
+    right-hand sides, a block the persistent path does not take)."""
     torch = torch_cuda
     disc, vel = _problem(16, 16, 20)
     o = _oracle(disc, vel, 5)
