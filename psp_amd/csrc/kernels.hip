@@ -235,27 +235,46 @@ __global__ __launch_bounds__(kThreads) void k_ldl_diag(const LdlItem* items, int
     const LdlItem it = items[start + blockIdx.x];
     cx<T>* F = L + (int64_t)blockIdx.y * fstore + it.l;
     const int ld = it.ld, j0 = it.j0, m = it.j1 - it.j0;
+    static_assert(NB == 32 && kThreads == 256, "each thread owns 2 x 2 elements of the block");
     __shared__ cx<T> A[NB][NB + 1];  // A[col][row]
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-        const int r = e % m, c = e / m;
-        if (r >= c) A[c][r] = F[(j0 + r) + (int64_t)(j0 + c) * ld];
-    }
+    // thread (tx, ty) owns rows tx, tx + 16 and columns ty, ty + 16
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int r = tx + 16 * a, c = ty + 16 * b;
+            if (r < m && c <= r) A[c][r] = F[(j0 + r) + (int64_t)(j0 + c) * ld];
+        }
     __syncthreads();
+    // Step j updates the trailing block with column j (unscaled) and scales
+    // column j - 1, which step j no longer reads: one barrier per column.
     for (int j = 0; j < m; ++j) {
         const cx<T> dinv = cinv(A[j][j]);
-        const int mm = m - j - 1;
-        for (int e = threadIdx.x; e < mm * mm; e += blockDim.x) {
-            const int r = j + 1 + e % mm, c = j + 1 + e / mm;
-            if (r >= c) A[c][r] = cfms(A[j][r] * dinv, A[j][c], A[c][r]);
+        if (j > 0 && ty == 0) {
+            const cx<T> dp = cinv(A[j - 1][j - 1]);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const int r = tx + 16 * a;
+                if (r > j - 1 && r < m) A[j - 1][r] = A[j - 1][r] * dp;
+            }
         }
-        __syncthreads();
-        for (int r = j + 1 + threadIdx.x; r < m; r += blockDim.x) A[j][r] = A[j][r] * dinv;
-        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int r = tx + 16 * a, c = ty + 16 * b;
+                if (c > j && r >= c && r < m) A[c][r] = cfms(A[j][r] * dinv, A[j][c], A[c][r]);
+            }
+        __syncthreads();  // no global loads are in flight here
     }
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-        const int r = e % m, c = e / m;
-        if (r >= c) F[(j0 + r) + (int64_t)(j0 + c) * ld] = A[c][r];
-    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int r = tx + 16 * a, c = ty + 16 * b;
+            if (r < m && c <= r) F[(j0 + r) + (int64_t)(j0 + c) * ld] = A[c][r];
+        }
 }
 
 // Blocked LDL^T, step 2: rows below the diagonal block,
@@ -872,7 +891,7 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
 // fields are made wave-uniform (so its base addresses are scalar), every
 // lane's offsets within a step are fixed for the launch, and factor entries
 // are used unmasked.  That relies on the solve storage: the triangles a
-// phase does not use are zero (k_zero_triangles), the factor and vector
+// phase does not use are zero (k_transpose_scale), the factor and vector
 // buffers end in kSolvePad padding columns / rows of zeros, so the entries
 // a partial step reads past its tile are finite and meet a zero in the
 // staged vector (the staged vector is masked), and rows past a partial
@@ -1193,24 +1212,6 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
             drain(d0, i1);
             break;
         }
-    }
-}
-
-// Zero the triangle of each front's top block that the solves never use (F:
-// above the diagonal of L's top block; B: below the diagonal of LT's), so
-// the persistent solve path can read factor entries without masks.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_zero_triangles(const NodeDev* nodes, cx<T>* L,
-                                                              cx<T>* LT, int64_t fstore)
-{
-    const NodeDev nd = nodes[blockIdx.x];
-    const int s = nd.s, fs = nd.s + nd.u;
-    cx<T>* Lf = L + (int64_t)blockIdx.y * fstore + nd.l;
-    cx<T>* Lt = LT + (int64_t)blockIdx.y * fstore + nd.l;
-    for (int64_t e = threadIdx.x; e < (int64_t)s * s; e += kThreads) {
-        const int i = (int)(e % s), j = (int)(e / s);
-        if (j > i) Lf[i + (int64_t)j * fs] = czero<T>();
-        if (j < i) Lt[i + (int64_t)j * s] = czero<T>();
     }
 }
 
@@ -1574,13 +1575,13 @@ __global__ __launch_bounds__(kThreads) void k_mv_reduce(const SolveTile* items, 
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_transpose_scale(const TransItem* items,
                                                                const NodeDev* nodes,
-                                                               const cx<T>* L, cx<T>* LT,
+                                                               cx<T>* L, cx<T>* LT,
                                                                int64_t fstore)
 {
     const TransItem it = items[blockIdx.x];
     const NodeDev nd = nodes[it.node];
     const int s = nd.s, fs = nd.s + nd.u;
-    const cx<T>* Lf = L + (int64_t)blockIdx.y * fstore + nd.l;
+    cx<T>* Lf = L + (int64_t)blockIdx.y * fstore + nd.l;
     cx<T>* Lt = LT + (int64_t)blockIdx.y * fstore + nd.l;
     __shared__ cx<T> tile[32][33];
     __shared__ cx<T> dinv[32];
@@ -1594,11 +1595,17 @@ __global__ __launch_bounds__(kThreads) void k_transpose_scale(const TransItem* i
         if (i < s && j < s) tile[jj][t32] = Lf[i + (int64_t)j * fs];
     }
     __syncthreads();
+    // LT's block, with zeros below its diagonal; L's block is read by now,
+    // and its part above the diagonal (which no solve uses) becomes zero
     for (int il = q; il < 32; il += 8) {
         const int j = it.j0 + t32, i = it.i0 + il;
         if (i < s && j < s)
             Lt[j + (int64_t)i * s] =
                 i > j ? tile[t32][il] * dinv[il] : (i == j ? dinv[il] : czero<T>());
+    }
+    for (int jj = q; jj < 32; jj += 8) {
+        const int i = it.i0 + t32, j = it.j0 + jj;
+        if (i < s && j < s && j > i) Lf[i + (int64_t)j * fs] = czero<T>();
     }
 }
 
@@ -2121,15 +2128,7 @@ void LaunchTransposeScale(const ClassDev& c, hipStream_t s)
 {
     if (c.ntrans <= 0) return;
     hipLaunchKernelGGL((k_transpose_scale<T>), dim3(c.ntrans, c.npanels), dim3(kThreads), 0, s,
-                       c.trans, c.nodes, (const cx<T>*)c.L, (cx<T>*)c.LT, c.fstore);
-}
-
-template <typename T>
-void LaunchZeroTriangles(const ClassDev& c, hipStream_t s)
-{
-    if (c.nnodes <= 0) return;
-    hipLaunchKernelGGL((k_zero_triangles<T>), dim3(c.nnodes, c.npanels), dim3(kThreads), 0, s,
-                       c.nodes, (cx<T>*)c.L, (cx<T>*)c.LT, c.fstore);
+                       c.trans, c.nodes, (cx<T>*)c.L, (cx<T>*)c.LT, c.fstore);
 }
 
 template <typename T>
@@ -2248,7 +2247,6 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
                                       void*, int, hipStream_t, MvProfile*, int, int);             \
     template void LaunchTransposeScale<T>(const ClassDev&, hipStream_t);                          \
     template void LaunchFoldBack<T>(const ClassDev&, hipStream_t);                                \
-    template void LaunchZeroTriangles<T>(const ClassDev&, hipStream_t);                           \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
                                   int, hipStream_t);                                              \
     template void LaunchCopyOut<T>(const ClassDev&, c128*, const void*, int, int, int, int, int,  \

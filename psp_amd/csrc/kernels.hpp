@@ -127,9 +127,7 @@ template <typename T>
 void LaunchTransposeScale(const ClassDev& c, hipStream_t s);
 template <typename T>
 void LaunchFoldBack(const ClassDev& c, hipStream_t s);
-// Zeroes the unused triangles of the fronts' top blocks in L and LT.
-template <typename T>
-void LaunchZeroTriangles(const ClassDev& c, hipStream_t s);
+
 // Natural-order <-> padded panel vector.  mode 0: B = Xp, 1: B += Xp.
 template <typename T>
 void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int ny, int nz,

@@ -506,7 +506,6 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv, dou
         LaunchTransposeScale<T>(D, s);
         LaunchGemm<T>(D, P.fold.tileStart, P.fold.tileCount, s);
         LaunchFoldBack<T>(D, s);
-        LaunchZeroTriangles<T>(D, s);
         HIPCHK(c, hipEventRecord(e[3], s));
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventSynchronize(e[3]));
