@@ -744,8 +744,6 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave / WK, wk = wave % WK;
     const int li = lane & 15, lk = lane >> 4;
-    const int arow = tl.row0 + 16 * wr + li;  // row of this lane's A entries
-    const bool arowOk = 16 * wr + li < tl.nrows;
     // Finishing lanes: wk == 0, li < 8; element (b, q) is row
     // 16 wr + row(lk, q) of the tile and right-hand side 8 b + li of the pass.
     const bool fin = wk == 0 && li < 8 && tl.slot < 0;
@@ -778,14 +776,20 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
             const int nsteps = (ncol + 3) >> 2;
             const int mine = nsteps > wk ? (nsteps - wk + WK - 1) / WK : 0;
             cx<T> m[U];
+            // Factor entries from a scalar base and a per-lane offset, rows
+            // clamped into the tile, unmasked: the unused triangles of the
+            // solve storage are zero, and a wave reads at most 3 columns past
+            // the chunk (inside the storage's zero padding), against zeros
+            // in the staged vector.  Steps past the wave's share reload its
+            // last one and are not multiplied.
+            const int fa = PH == PH_F ? fs : s;
+            const cx<T>* Ab = (PH == PH_F ? Lf : Lt) + tl.row0 + (int64_t)cb * fa;
+            const uint32_t la = (uint32_t)(min(16 * wr + li, tl.nrows - 1) + (4 * wk + lk) * fa);
+            const int64_t ustep = (int64_t)(4 * WK) * fa;
+            const int ulast = max(mine - 1, 0);
             auto load = [&](cx<T>* dst, int u0) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int c = 4 * (wk + WK * (u0 + u)) + lk;
-                    dst[u] = (arowOk && u0 + u < mine && c < ncol)
-                                 ? FactorEntry<T, PH>(Lf, Lt, s, fs, arow, cb + c)
-                                 : czero<T>();
-                }
+                for (int u = 0; u < U; ++u) dst[u] = (Ab + min(u0 + u, ulast) * ustep)[la];
             };
             // staging loads first, then the first factor batch, all in flight
             // together; the LDS stores wait only for the staging loads
