@@ -1670,7 +1670,8 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_copy_out(int n, int k, const int32_t* qxy,
                                                         const int32_t* qv, c128* B,
                                                         const cx<T>* Xp, int nx, int ny, int nz,
-                                                        int vOffset, int padding, int mode)
+                                                        int vOffset, int padding, int mode,
+                                                        StencilArgs st, const double* vel, int v1)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * k) return;
@@ -1685,47 +1686,44 @@ __global__ __launch_bounds__(kThreads) void k_copy_out(int n, int k, const int32
         B[nat * k + r] = v;
     else
         B[nat * k + r] = B[nat * k + r] + v;
+    // forward sweep: the subdiagonal coupling into the next panel's first
+    // plane v1 (k_subdiag's update), applied by the thread that wrote the
+    // value it reads: row (x,y,v1) couples to (x,y,v1-1) with value c[5]
+    if (v1 >= 0 && vOffset + vl == v1 - 1) {
+        const int64_t natN = nat - (int64_t)nx * ny;
+        c128 c[7];
+        GlobalRowFast(st, vel[natN], x, y, v1, c);
+        B[natN * k + r] = B[natN * k + r] - c[5] * v;
+    }
 }
 
 // --------------------------------------------------------------------------
 // Sweep couplings between consecutive panels (solve.hpp:884-1047).
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_subdiag(StencilArgs st, const double* vel, c128* B,
-                                                       int k, int v1)
-{
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t plane = (int64_t)st.nx * st.ny;
-    if (e >= plane * k) return;
-    const int64_t xy = e / k;
-    const int r = (int)(e - xy * k);
-    const int x = (int)(xy % st.nx), y = (int)(xy / st.nx);
-    const int z = st.nz - 1 - v1;
-    const int64_t nat = xy + (int64_t)z * plane;
-    c128 c[7];
-    GlobalRowFast(st, vel[nat], x, y, v1, c);
-    // row (x,y,v1) couples to (x,y,v1-1) = natural z+1 with value c[5]
-    B[nat * k + r] = B[nat * k + r] - c[5] * B[(nat + plane) * k + r];
-}
-
+// Backward sweep (ExtractPanel + MultiplySuperdiagonal): the panel vector is
+// zero except on its last local plane vlLast (global plane v2), which holds
+// -A(v2, v2+1) B(plane v2+1), i.e. row (x,y,v2) times its coupling c[6] to
+// (x,y,v2+1) = natural z-1; one pass over the panel vector.  The forward
+// sweep's subdiagonal coupling is applied by k_copy_out.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_supdiag_set(StencilArgs st, const double* vel,
-                                                           const c128* B, const int32_t* nat2q,
-                                                           cx<T>* Xp, int k, int v2, int vlLast)
+__global__ __launch_bounds__(kThreads) void k_zero_set_plane(int n, int k, const int32_t* qxy,
+                                                              const int32_t* qv, StencilArgs st,
+                                                              const double* vel, const c128* B,
+                                                              cx<T>* Xp, int v2, int vlLast)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t plane = (int64_t)st.nx * st.ny;
-    if (e >= plane * k) return;
-    const int64_t xy = e / k;
-    const int r = (int)(e - xy * k);
-    const int x = (int)(xy % st.nx), y = (int)(xy / st.nx);
-    const int z = st.nz - 1 - v2;
-    const int64_t nat = xy + (int64_t)z * plane;
-    c128 c[7];
-    GlobalRowFast(st, vel[nat], x, y, v2, c);
-    // row (x,y,v2) couples to (x,y,v2+1) = natural z-1 with value c[6]
-    const c128 val = -(c[6] * B[(nat - plane) * k + r]);
-    const int q = nat2q[xy + (int64_t)vlLast * plane];
-    Xp[(int64_t)q * k + r] = cvt<T>(val);
+    if (e >= (int64_t)n * k) return;
+    const int q = (int)(e / k), r = (int)(e - (int64_t)q * k);
+    cx<T> out = czero<T>();
+    if (qv[q] == vlLast) {
+        const int x = qxy[q] & 0xffff, y = qxy[q] >> 16;
+        const int64_t plane = (int64_t)st.nx * st.ny;
+        const int64_t nat = x + (int64_t)y * st.nx + (int64_t)(st.nz - 1 - v2) * plane;
+        c128 c[7];
+        GlobalRowFast(st, vel[nat], x, y, v2, c);
+        out = cvt<T>(-(c[6] * B[(nat - plane) * k + r]));
+    }
+    Xp[e] = out;
 }
 
 // --------------------------------------------------------------------------
@@ -2153,26 +2151,21 @@ void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int
 
 template <typename T>
 void LaunchCopyOut(const ClassDev& c, c128* B, const void* Xp, int k, int nx, int ny, int nz,
-                   int vOffset, int padding, int mode, hipStream_t s)
+                   int vOffset, int padding, int mode, hipStream_t s, const StencilArgs* st,
+                   const double* vel, int v1)
 {
+    const StencilArgs none{};
     hipLaunchKernelGGL((k_copy_out<T>), dim3(Blocks((int64_t)c.n * k)), dim3(kThreads), 0, s, c.n,
-                       k, c.qxy, c.qv, B, (const cx<T>*)Xp, nx, ny, nz, vOffset, padding, mode);
-}
-
-void LaunchSubdiag(const StencilArgs& st, const double* vel, c128* B, int k, int v1,
-                   hipStream_t s)
-{
-    const int64_t n = (int64_t)st.nx * st.ny * k;
-    hipLaunchKernelGGL(k_subdiag, dim3(Blocks(n)), dim3(kThreads), 0, s, st, vel, B, k, v1);
+                       k, c.qxy, c.qv, B, (const cx<T>*)Xp, nx, ny, nz, vOffset, padding, mode,
+                       st ? *st : none, vel, st ? v1 : -1);
 }
 
 template <typename T>
-void LaunchSupdiagSet(const ClassDev& c, const StencilArgs& st, const double* vel,
-                      const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s)
+void LaunchZeroSetPlane(const ClassDev& c, const StencilArgs& st, const double* vel,
+                        const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s)
 {
-    const int64_t n = (int64_t)st.nx * st.ny * k;
-    hipLaunchKernelGGL((k_supdiag_set<T>), dim3(Blocks(n)), dim3(kThreads), 0, s, st, vel, B,
-                       c.nat2q, (cx<T>*)Xp, k, v2, vlLast);
+    hipLaunchKernelGGL((k_zero_set_plane<T>), dim3(Blocks((int64_t)c.n * k)), dim3(kThreads), 0, s,
+                       c.n, k, c.qxy, c.qv, st, vel, B, (cx<T>*)Xp, v2, vlLast);
 }
 
 void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c128* b, c128* y,
@@ -2254,9 +2247,10 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
                                   int, hipStream_t);                                              \
     template void LaunchCopyOut<T>(const ClassDev&, c128*, const void*, int, int, int, int, int,  \
-                                   int, int, hipStream_t);                                        \
-    template void LaunchSupdiagSet<T>(const ClassDev&, const StencilArgs&, const double*,         \
-                                      const c128*, void*, int, int, int, hipStream_t);
+                                   int, int, hipStream_t, const StencilArgs*, const double*,      \
+                                   int);                                                          \
+    template void LaunchZeroSetPlane<T>(const ClassDev&, const StencilArgs&, const double*,       \
+                                        const c128*, void*, int, int, int, hipStream_t);
 PSP_INST(double)
 PSP_INST(float)
 #undef PSP_INST

@@ -134,14 +134,13 @@ void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int
                   int vOffset, int padding, hipStream_t s);
 template <typename T>
 void LaunchCopyOut(const ClassDev& c, c128* B, const void* Xp, int k, int nx, int ny, int nz,
-                   int vOffset, int padding, int mode, hipStream_t s);
-// B_{i+1}(first plane v1) -= A(v1, v1-1) B_i(plane v1-1)
-void LaunchSubdiag(const StencilArgs& st, const double* vel, c128* B, int k, int v1,
-                   hipStream_t s);
-// Xp(last plane of panel i) := -A(v2, v2+1) B(plane v2+1); rest of Xp is 0.
+                   int vOffset, int padding, int mode, hipStream_t s,
+                   const StencilArgs* st = nullptr, const double* vel = nullptr, int v1 = -1);
+// Backward sweep, one pass: Xp = 0 except the superdiagonal coupling on the
+// panel's last local plane (MultiplySuperdiagonal, solve.hpp:725-733).
 template <typename T>
-void LaunchSupdiagSet(const ClassDev& c, const StencilArgs& st, const double* vel,
-                      const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s);
+void LaunchZeroSetPlane(const ClassDev& c, const StencilArgs& st, const double* vel,
+                        const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s);
 
 // y = A x (mode 0) or y = b - A x (mode 1), natural order.
 void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c128* b,

@@ -534,17 +534,22 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv, dou
 // ---------------------------------------------------------------------------
 // Sweep: Precondition (solve.hpp:681-734) on a natural-order device block.
 // ---------------------------------------------------------------------------
+// couple >= 0: the copy-out also applies the subdiagonal coupling into the
+// next panel's first plane `couple` (SolvePanel + MultiplySubdiagonal).
 template <typename T>
-void PanelForward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = nullptr)
+void PanelForward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = nullptr,
+                  int couple = -1)
 {
     const Geometry& g = c->plan.geo;
     const int cl = g.cls(p);
     const ClassDev& D = c->dev[cl];
     hipStream_t s = c->stream;
+    const StencilArgs st = Stencil(c);
     LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s);
     LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, c->G[cl], k, s,
                         prof);
-    LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s);
+    LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s,
+                     couple >= 0 ? &st : nullptr, c->vel, couple);
 }
 
 template <typename T>
@@ -557,10 +562,10 @@ void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = null
     const int cl = g.cls(p);
     const ClassDev& D = c->dev[cl];
     hipStream_t s = c->stream;
-    LaunchFillZero(c->Xp, (int64_t)sizeof(cx<T>) * D.n * k, s);
+    // Xp = 0 except the superdiagonal coupling on the panel's last plane
     const int v2 = g.panelV(p) + g.depth(p) - 1;
     const int vlLast = g.padding(p) + g.depth(p) - 1;
-    LaunchSupdiagSet<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
+    LaunchZeroSetPlane<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
     LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, c->G[cl], k, s,
                         prof);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 1, s);
@@ -570,11 +575,7 @@ template <typename T>
 void Sweep(psp_hip_ctx* c, c128* B, int k, MvProfile* prof = nullptr)
 {
     const Geometry& g = c->plan.geo;
-    const StencilArgs st = Stencil(c);
-    for (int i = 0; i < g.numPanels - 1; ++i) {
-        PanelForward<T>(c, B, k, i, prof);
-        LaunchSubdiag(st, c->vel, B, k, g.panelV(i + 1), c->stream);
-    }
+    for (int i = 0; i < g.numPanels - 1; ++i) PanelForward<T>(c, B, k, i, prof, g.panelV(i + 1));
     PanelForward<T>(c, B, k, g.numPanels - 1, prof);
     for (int i = g.numPanels - 2; i >= 0; --i) PanelBackward<T>(c, B, k, i, prof);
 }
