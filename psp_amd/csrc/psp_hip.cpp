@@ -678,6 +678,23 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
     std::vector<double> cs((size_t)m * k);
     std::vector<c128> hc((size_t)(m + 1) * k), yh((size_t)(m + 1) * k);
     int rc;
+    // Phase times from events on the stream, read at the host waits the
+    // method needs (the Hessenberg column, the residual norms), so the
+    // operator, the sweep and the Arnoldi steps queue back to back.
+    struct Events {
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    for (hipEvent_t& x : ev.e) HIPCHK(c, hipEventCreate(&x));
+    auto span = [&](int a, int b) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev.e[a], ev.e[b]);
+        return ms * 1e-3;
+    };
 
     LaunchFillZero(c->x, sizeof(c128) * n, s);
     HIPCHK(c, hipMemcpyAsync(c->w, c->b, sizeof(c128) * n, hipMemcpyDeviceToDevice, s));
@@ -698,17 +715,11 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
         for (int r = 0; r < k; ++r) z[(size_t)r * (m + 1)] = beta[r];
         for (int j = 0; j < m; ++j) {
             c128* vj = c->V + (int64_t)j * n;
-            t0 = Now();
+            HIPCHK(c, hipEventRecord(ev.e[0], s));
             LaunchSpmv(st, c->vel, vj, nullptr, c->w, k, 0, s);
-            HIPCHK(c, hipStreamSynchronize(s));
-            o.tM += Now() - t0;
-            ++o.nM;
-            t0 = Now();
+            HIPCHK(c, hipEventRecord(ev.e[1], s));
             if ((rc = Precondition(c, c->w, k))) return rc;
-            HIPCHK(c, hipStreamSynchronize(s));
-            o.tP += Now() - t0;
-            ++o.nP;
-            t0 = Now();
+            HIPCHK(c, hipEventRecord(ev.e[2], s));
             // Arnoldi by modified Gram-Schmidt (solve.hpp:382-399)
             for (int i = 0; i <= j; ++i) {
                 const c128* vi = c->V + (int64_t)i * n;
@@ -718,8 +729,13 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
             if ((rc = Norms(c, c->w, k, delta))) return rc;
             HIPCHK(c, hipMemcpyAsync(hc.data(), c->Hcol, sizeof(c128) * (j + 1) * k,
                                      hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipEventRecord(ev.e[3], s));
             HIPCHK(c, hipStreamSynchronize(s));
-            o.tA += Now() - t0;
+            o.tM += span(0, 1);
+            ++o.nM;
+            o.tP += span(1, 2);
+            ++o.nP;
+            o.tA += span(2, 3);
             if (HasNaN(delta)) return Fail(c, PSP_HIP_ERR_NAN, "delta list had a NaN");
             for (int r = 0; r < k; ++r)
                 if (delta[r] == 0.0)
@@ -764,12 +780,12 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
             HIPCHK(c, hipMemcpyAsync(c->ydev, yh.data(), sizeof(c128) * (j + 1) * k,
                                      hipMemcpyHostToDevice, s));
             LaunchUpdateX(c->x0, c->V, c->ydev, j, N, k, c->x, s);  // x := x0 + V y
-            t0 = Now();
+            HIPCHK(c, hipEventRecord(ev.e[0], s));
             LaunchSpmv(st, c->vel, c->x, c->b, c->w, k, 1, s);  // w := b - A x
-            HIPCHK(c, hipStreamSynchronize(s));
-            o.tM += Now() - t0;
+            HIPCHK(c, hipEventRecord(ev.e[1], s));
+            if ((rc = Norms(c, c->w, k, resid))) return rc;  // waits for the stream
+            o.tM += span(0, 1);
             ++o.nM;
-            if ((rc = Norms(c, c->w, k, resid))) return rc;
             if (HasNaN(resid)) return Fail(c, PSP_HIP_ERR_NAN, "resid norm list has NaN");
             double mx = 0;
             for (int r = 0; r < k; ++r) mx = std::max(mx, resid[r] / orig[r]);
