@@ -1856,6 +1856,41 @@ __global__ __launch_bounds__(kThreads) void k_coldot_part(const c128* V, const c
     }
 }
 
+// Modified Gram-Schmidt, two steps in one pass: W -= alpha V (step i's
+// update) and the partial sums of conj(Vn) W for step i + 1, with the
+// block partition and summation order of k_coldot_part.
+__global__ __launch_bounds__(kThreads) void k_axpy_dot_part(const c128* alpha, const c128* V,
+                                                             const c128* Vn, c128* W, int64_t np,
+                                                             int k, c128* part)
+{
+    __shared__ c128 sh[kThreads];
+    const int lanes = (kThreads / k) * k, per = kThreads / k;
+    const int t = threadIdx.x;
+    const int64_t chunk = (np + gridDim.x - 1) / gridDim.x;
+    const int64_t p0 = (int64_t)blockIdx.x * chunk, p1 = min(np, p0 + chunk);
+    c128 acc = {0.0, 0.0};
+    if (t < lanes) {
+        const int r = t % k;
+        const c128 a = alpha[r];
+        for (int64_t p = p0 + t / k; p < p1; p += per) {
+            const int64_t e = p * k + r;
+            const c128 w = cfms(a, V[e], W[e]), v = Vn[e];
+            W[e] = w;
+            acc.r = fma(v.r, w.r, acc.r);
+            acc.r = fma(v.i, w.i, acc.r);
+            acc.i = fma(v.r, w.i, acc.i);
+            acc.i = fma(-v.i, w.r, acc.i);
+        }
+    }
+    sh[t] = acc;
+    __syncthreads();
+    if (t < k) {
+        c128 s = {0.0, 0.0};
+        for (int i = 0; i < per; ++i) s = s + sh[t + i * k];
+        part[(int64_t)blockIdx.x * k + t] = s;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_coldot_final(const c128* part, int nblk, int k,
                                                             c128* out)
 {
@@ -2192,6 +2227,14 @@ void LaunchColDot(const c128* v, const c128* w, int64_t np, int k, c128* part, c
 {
     const int g = ReducePartials(np);
     hipLaunchKernelGGL(k_coldot_part, dim3(g), dim3(kThreads), 0, s, v, w, np, k, part);
+    hipLaunchKernelGGL(k_coldot_final, dim3(k), dim3(kThreads), 0, s, part, g, k, out);
+}
+
+void LaunchAxpyDot(const c128* alpha, const c128* v, const c128* vn, c128* w, int64_t np, int k,
+                   c128* part, c128* out, hipStream_t s)
+{
+    const int g = ReducePartials(np);
+    hipLaunchKernelGGL(k_axpy_dot_part, dim3(g), dim3(kThreads), 0, s, alpha, v, vn, w, np, k, part);
     hipLaunchKernelGGL(k_coldot_final, dim3(k), dim3(kThreads), 0, s, part, g, k, out);
 }
 

@@ -152,6 +152,10 @@ void LaunchColNorm2(const c128* x, int64_t npoints, int k, double* part, double*
                     hipStream_t s);  // out[r] = ||x_r||_2
 void LaunchColDot(const c128* v, const c128* w, int64_t npoints, int k, c128* part,
                   c128* out, hipStream_t s);  // out[r] = v_r^H w_r
+// w -= alpha v, then out = conj(vn)^T w per column (one pass over w; the
+// same sums as LaunchAxpyCols followed by LaunchColDot).
+void LaunchAxpyDot(const c128* alpha, const c128* v, const c128* vn, c128* w, int64_t np, int k,
+                   c128* part, c128* out, hipStream_t s);
 void LaunchAxpyCols(const c128* alpha, int sign, const c128* v, c128* w, int64_t npoints,
                     int k, hipStream_t s);  // w += sign*alpha_r v_r
 void LaunchScaleCols(const double* delta, const c128* w, c128* v, int64_t npoints, int k,

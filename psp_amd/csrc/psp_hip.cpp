@@ -720,12 +720,14 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
             HIPCHK(c, hipEventRecord(ev.e[1], s));
             if ((rc = Precondition(c, c->w, k))) return rc;
             HIPCHK(c, hipEventRecord(ev.e[2], s));
-            // Arnoldi by modified Gram-Schmidt (solve.hpp:382-399)
-            for (int i = 0; i <= j; ++i) {
-                const c128* vi = c->V + (int64_t)i * n;
-                LaunchColDot(vi, c->w, N, k, c->partC, c->Hcol + (int64_t)i * k, s);
-                LaunchAxpyCols(c->Hcol + (int64_t)i * k, -1, vi, c->w, N, k, s);
-            }
+            // Arnoldi by modified Gram-Schmidt (solve.hpp:382-399): step i's
+            // update of w and step i + 1's dot product share one pass
+            LaunchColDot(c->V, c->w, N, k, c->partC, c->Hcol, s);
+            for (int i = 0; i < j; ++i)
+                LaunchAxpyDot(c->Hcol + (int64_t)i * k, c->V + (int64_t)i * n,
+                              c->V + (int64_t)(i + 1) * n, c->w, N, k, c->partC,
+                              c->Hcol + (int64_t)(i + 1) * k, s);
+            LaunchAxpyCols(c->Hcol + (int64_t)j * k, -1, c->V + (int64_t)j * n, c->w, N, k, s);
             if ((rc = Norms(c, c->w, k, delta))) return rc;
             HIPCHK(c, hipMemcpyAsync(hc.data(), c->Hcol, sizeof(c128) * (j + 1) * k,
                                      hipMemcpyDeviceToHost, s));
