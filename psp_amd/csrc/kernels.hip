@@ -399,29 +399,6 @@ struct RawRow {  // gathered value of a bottom row (two slot copies) and its par
     int32_t prow;
 };
 
-// Forward input of front row r: the panel right-hand side on the node's own
-// rows plus the children's update vectors.
-template <typename T>
-__device__ __forceinline__ cx<T> Gathered(const NodeDev& nd, const Slots<T>& gs, const cx<T>* Xp,
-                                          int r, int rr, int k)
-{
-    const int64_t g = (nd.w + r) * k + rr;
-    cx<T> v = gs.G[g] + gs.G[gs.half + g];
-    if (r < nd.s) v = v + Xp[(int64_t)(nd.off + r) * k + rr];
-    return v;
-}
-
-// Inner operand of a phase at inner index col, right-hand side rr.
-template <typename T, int PH>
-__device__ __forceinline__ cx<T> InnerOperand(const NodeDev& nd, const int32_t* structs,
-                                              const Slots<T>& gs, const cx<T>* Xp,
-                                              const cx<T>* W, int col, int rr, int k)
-{
-    if (PH == PH_F) return Gathered(nd, gs, Xp, col, rr, k);
-    if (col < nd.s) return W[(nd.w + col) * k + rr];
-    return Xp[(int64_t)structs[nd.ustart + col - nd.s] * k + rr];
-}
-
 // The value v, passed through an empty asm: the compiler must hold it in a
 // register at this point, so a load that produced it is not sunk into the
 // arm of a later select (where its wait would sit inside a branch, one
