@@ -326,8 +326,11 @@ int EnsureDevice(psp_hip_ctx* c)
         const size_t fb = eb * ((size_t)D.npanels * D.fstore + (size_t)kSolvePad * maxFs);
         if ((st = Alloc(c, &D.L, fb, c->allocs))) return st;
         if ((st = Alloc(c, &D.LT, fb, c->allocs))) return st;
-        HIPCHK(c, hipMemsetAsync(D.L, 0, fb, c->stream));
-        HIPCHK(c, hipMemsetAsync(D.LT, 0, fb, c->stream));
+        // only the padding needs zeroing here: every factorization zeroes L
+        // before assembly, and the fold writes every entry of LT
+        const size_t body = eb * (size_t)D.npanels * D.fstore;
+        HIPCHK(c, hipMemsetAsync((char*)D.L + body, 0, fb - body, c->stream));
+        HIPCHK(c, hipMemsetAsync((char*)D.LT + body, 0, fb - body, c->stream));
     }
     c->devReady = true;
     return PSP_HIP_OK;
