@@ -117,3 +117,22 @@ def test_python_api_mirrors_reference_names():
     d = psp_amd.Discretization(1.0, 4, 4, 4, 1.0, 2.0, 1.0)
     assert d.sigmax == pytest.approx(2.4)  # 1.2 * max width (discretization.hpp:60-63)
     assert (d.bx, d.by, d.bz) == (5, 5, 5)
+
+
+def test_solve_tiling_options_checked_on_host():
+    """The front-solve tiling options (default shapes, persistent workgroups
+    and their grid cap) rebuild the tile lists on the host before any device
+    work, and out-of-range values are rejected with the library's message."""
+    import psp_amd
+    from psp_amd import workloads
+    disc = workloads.discretization(12, 10, 24, 1.0, 10.0)
+    h = psp_amd.DistHelmholtz(disc, cutoff=5)
+    h.set_solve_tiling(persist=False)
+    h.set_solve_tiling(persist=True, persist_grid=4, min_tiles=64, max_split=2, wide_wg=True)
+    h.set_solve_tiling(persist_grid=0)
+    with pytest.raises(psp_amd.LogicError, match="persistent grid cap"):
+        h.set_solve_tiling(persist_grid=-1)
+    with pytest.raises(psp_amd.LogicError, match="max split"):
+        h.set_solve_tiling(max_split=0)
+    with pytest.raises(psp_amd.LogicError, match="min tiles"):
+        h.set_solve_tiling(min_tiles=0)
