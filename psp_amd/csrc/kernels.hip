@@ -1804,8 +1804,10 @@ __global__ __launch_bounds__(kThreads) void k_colnorm_final(const double* part, 
     if (threadIdx.x == 0) out[r] = sqrt(sh[0]);
 }
 
-__global__ __launch_bounds__(kThreads) void k_coldot_part(const c128* V, const c128* W,
-                                                           int64_t np, int k, c128* part)
+__global__ __launch_bounds__(kThreads) void k_coldot_part(const c128* __restrict__ V,
+                                                           const c128* __restrict__ W,
+                                                           int64_t np, int k,
+                                                           c128* __restrict__ part)
 {
     __shared__ c128 sh[kThreads];
     const int lanes = (kThreads / k) * k, per = kThreads / k;
@@ -1815,6 +1817,7 @@ __global__ __launch_bounds__(kThreads) void k_coldot_part(const c128* V, const c
     c128 acc = {0.0, 0.0};
     if (t < lanes) {
         const int r = t % k;
+#pragma unroll 4
         for (int64_t p = p0 + t / k; p < p1; p += per) {
             const c128 v = V[p * k + r], w = W[p * k + r];
             // conj(v) * w
@@ -1836,9 +1839,11 @@ __global__ __launch_bounds__(kThreads) void k_coldot_part(const c128* V, const c
 // Modified Gram-Schmidt, two steps in one pass: W -= alpha V (step i's
 // update) and the partial sums of conj(Vn) W for step i + 1, with the
 // block partition and summation order of k_coldot_part.
-__global__ __launch_bounds__(kThreads) void k_axpy_dot_part(const c128* alpha, const c128* V,
-                                                             const c128* Vn, c128* W, int64_t np,
-                                                             int k, c128* part)
+__global__ __launch_bounds__(kThreads) void k_axpy_dot_part(const c128* __restrict__ alpha,
+                                                             const c128* __restrict__ V,
+                                                             const c128* __restrict__ Vn,
+                                                             c128* __restrict__ W, int64_t np,
+                                                             int k, c128* __restrict__ part)
 {
     __shared__ c128 sh[kThreads];
     const int lanes = (kThreads / k) * k, per = kThreads / k;
@@ -1849,6 +1854,9 @@ __global__ __launch_bounds__(kThreads) void k_axpy_dot_part(const c128* alpha, c
     if (t < lanes) {
         const int r = t % k;
         const c128 a = alpha[r];
+        // unrolled so a thread has several elements' loads in flight (the
+        // sums still run in element order)
+#pragma unroll 4
         for (int64_t p = p0 + t / k; p < p1; p += per) {
             const int64_t e = p * k + r;
             const c128 w = cfms(a, V[e], W[e]), v = Vn[e];
