@@ -1706,28 +1706,32 @@ __global__ __launch_bounds__(kThreads) void k_zero_set_plane(int n, int k, const
 // --------------------------------------------------------------------------
 // Multiply (solve.hpp:613-679) in natural order: y = A x or y = b - A x.
 // --------------------------------------------------------------------------
+// I: index type, uint32_t when every element index (and the grid's last
+// thread) fits, which keeps the index divisions 32-bit; int64_t otherwise.
+template <typename I>
 __global__ __launch_bounds__(kThreads) void k_spmv(StencilArgs st, const double* vel,
                                                     const c128* X, const c128* Bv, c128* Y, int k,
                                                     int mode)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t plane = (int64_t)st.nx * st.ny;
-    const int64_t N = plane * st.nz;
-    if (e >= N * k) return;
-    const int64_t p = e / k;
-    const int x = (int)(p % st.nx);
-    const int y = (int)((p / st.nx) % st.ny);
-    const int z = (int)(p / plane);
+    const I e = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
+    const I nx = (I)st.nx, ny = (I)st.ny, kk = (I)k;
+    const I plane = nx * ny;
+    if (e >= plane * (I)st.nz * kk) return;
+    const I p = e / kk;
+    const I py = p / nx;
+    const int x = (int)(p - py * nx);
+    const int y = (int)(py % ny);
+    const int z = (int)(py / ny);
     const int v = st.nz - 1 - z;
     c128 c[7];
     GlobalRowFast(st, vel[p], x, y, v, c);
     c128 acc = X[e] * c[0];
-    if (x > 0) acc = cfma(c[1], X[e - k], acc);
-    if (x + 1 < st.nx) acc = cfma(c[2], X[e + k], acc);
-    if (y > 0) acc = cfma(c[3], X[e - (int64_t)st.nx * k], acc);
-    if (y + 1 < st.ny) acc = cfma(c[4], X[e + (int64_t)st.nx * k], acc);
-    if (v > 0) acc = cfma(c[5], X[e + plane * k], acc);
-    if (v + 1 < st.nz) acc = cfma(c[6], X[e - plane * k], acc);
+    if (x > 0) acc = cfma(c[1], X[e - kk], acc);
+    if (x + 1 < st.nx) acc = cfma(c[2], X[e + kk], acc);
+    if (y > 0) acc = cfma(c[3], X[e - nx * kk], acc);
+    if (y + 1 < st.ny) acc = cfma(c[4], X[e + nx * kk], acc);
+    if (v > 0) acc = cfma(c[5], X[e + plane * kk], acc);
+    if (v + 1 < st.nz) acc = cfma(c[6], X[e - plane * kk], acc);
     Y[e] = mode == 0 ? acc : (Bv[e] - acc);
 }
 
@@ -2192,7 +2196,12 @@ void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c
                 int k, int mode, hipStream_t s)
 {
     const int64_t n = (int64_t)st.nx * st.ny * st.nz * k;
-    hipLaunchKernelGGL(k_spmv, dim3(Blocks(n)), dim3(kThreads), 0, s, st, vel, x, b, y, k, mode);
+    if ((int64_t)Blocks(n) * kThreads <= (int64_t)UINT32_MAX)
+        hipLaunchKernelGGL((k_spmv<uint32_t>), dim3(Blocks(n)), dim3(kThreads), 0, s, st, vel, x,
+                           b, y, k, mode);
+    else
+        hipLaunchKernelGGL((k_spmv<int64_t>), dim3(Blocks(n)), dim3(kThreads), 0, s, st, vel, x, b,
+                           y, k, mode);
 }
 
 int ReducePartials(int64_t npoints)
