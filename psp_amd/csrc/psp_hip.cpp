@@ -863,7 +863,7 @@ namespace {
 
 // Tile-shape tuner for the front solves: for every panel class, level and
 // phase, times the launch (products + reduce) of each candidate shape on the
-// class's first panel and keeps the fastest.  The estimate of one sweep's
+// class's panels (one launch per panel, cold factors) and keeps the fastest.  The estimate of one sweep's
 // product time weighs each class by its panel solves per sweep (2P - 1 in
 // all: every panel forward, every panel but the last backward).
 template <typename T>
@@ -896,10 +896,14 @@ int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
             LaunchFillZero(c->Xp, (int64_t)(eb_ * P.n * k), s);
             LaunchFillZero(c->W, (int64_t)(eb_ * P.wstore * k), s);
             LaunchPanelSolve<T>(D, P, 0, c->Xp, c->W, c->P, c->G[cl], k, s, nullptr, d, ph);  // warm
+            // each repetition applies another panel's factors (where the
+            // class has several), so they come from HBM as in a sweep rather
+            // than from the caches the previous repetition filled
             constexpr int kReps = 3;
             HIPCHK(c, hipEventRecord(ea, s));
             for (int r = 0; r < kReps; ++r)
-                LaunchPanelSolve<T>(D, P, 0, c->Xp, c->W, c->P, c->G[cl], k, s, nullptr, d, ph);
+                LaunchPanelSolve<T>(D, P, (r + 1) % D.npanels, c->Xp, c->W, c->P, c->G[cl], k, s,
+                                    nullptr, d, ph);
             HIPCHK(c, hipEventRecord(eb, s));
             HIPCHK(c, hipEventSynchronize(eb));
             HIPCHK(c, hipGetLastError());
