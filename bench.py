@@ -246,8 +246,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the whole grid lives in one "
-                          "MI355X's HBM; multi-GPU sharding is a design only (see DESIGN.md)",
+        print(json.dumps({"error": "psp_amd runs on one GPU: the whole grid lives in one "
+                          "MI355X's HBM; sharding over several GPUs is not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -79,7 +79,7 @@ public:
     void Solve(DistUniformGrid<C>& B, int m = 20, Real relTol = 1e-4,
                bool viewIterates = false) const
     {
-        (void)viewIterates;  // VTK iterate dumps are not part of this build
+        (void)viewIterates;  // VTK iterate dumps are not written
         const int st = psp_hip_solve(ctx_, B.Buffer(), B.NumScalars(), m, relTol, 0, 0,
                                      &solveReport_);
         ThrowOnStatus(st, ctx_, "Solve");
