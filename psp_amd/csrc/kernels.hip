@@ -1459,7 +1459,7 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
 // Batched tile GEMM on the matrix cores: C (+)= +/- opA(A) diag(D) opB(B)
 // over a list of 64x64 tiles (see GemmFlags), through the tile engine.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_mma(const GemmDesc* descs, const GemmTile* tiles,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 1) void k_gemm_mma(const GemmDesc* descs, const GemmTile* tiles,
                                                   int tileStart, cx<T>* b0, int64_t st0,
                                                   cx<T>* b1, int64_t st1, cx<T>* b2,
                                                   int64_t st2, cx<T>* b3)
