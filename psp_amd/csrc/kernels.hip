@@ -1459,12 +1459,12 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
 // Batched tile GEMM on the matrix cores: C (+)= +/- opA(A) diag(D) opB(B)
 // over a list of 64x64 tiles (see GemmFlags), through the tile engine.
 template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 1) void k_gemm_mma(const GemmDesc* descs, const GemmTile* tiles,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_gemm_mma(const GemmDesc* descs, const GemmTile* tiles,
                                                   int tileStart, cx<T>* b0, int64_t st0,
                                                   cx<T>* b1, int64_t st1, cx<T>* b2,
                                                   int64_t st2, cx<T>* b3)
 {
-    constexpr int BK = sizeof(T) == 4 ? 8 : kBK;  // inner columns per stage
+    constexpr int BK = 8;  // inner columns per stage
     constexpr int TS = 64;   // 128x128 tiles measured slower on the Schur updates
     __shared__ cx<T> As[2][BK][TS + 1];
     __shared__ cx<T> Bs[2][BK][TS + 1];
