@@ -1458,6 +1458,9 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
 
 // Batched tile GEMM on the matrix cores: C (+)= +/- opA(A) diag(D) opB(B)
 // over a list of 64x64 tiles (see GemmFlags), through the tile engine.
+// 8-column LDS stages keep the registers low enough for 5 waves per SIMD in
+// FP32 and 3 in FP64 without spills; on the factorization's many short-K
+// Schur updates that occupancy is worth more than deeper stages.
 template <typename T>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_gemm_mma(const GemmDesc* descs, const GemmTile* tiles,
                                                   int tileStart, cx<T>* b0, int64_t st0,
