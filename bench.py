@@ -303,7 +303,7 @@ def main():
         bufs = [Bsrc.clone() for _ in range(max(steps, 1))]
         for w in range(warmup):
             x = Bsrc.clone()
-            r = h.Solve(x, args.m, args.tol, args.max_its, raise_on_error=False)
+            r = h.Solve(x, args.m, args.tol, maxIts=args.max_its, raise_on_error=False)
             log(f"[bench] {precision} warmup {w}: {r.iterations} its, rel {r.final_max_rel:.2e}, "
                 f"{r.seconds_total:.3f}s (precond {r.seconds_precond:.3f}, "
                 f"multiply {r.seconds_multiply:.3f}, arnoldi {r.seconds_arnoldi:.3f})")
@@ -311,7 +311,7 @@ def main():
         t0 = time.perf_counter()
         reps = []
         for s in range(steps):
-            reps.append(h.Solve(bufs[s], args.m, args.tol, args.max_its, raise_on_error=False))
+            reps.append(h.Solve(bufs[s], args.m, args.tol, maxIts=args.max_its, raise_on_error=False))
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         for s, r in enumerate(reps):

@@ -132,8 +132,21 @@ typedef struct psp_hip_ctx psp_hip_ctx;
  * touched until psp_hip_factor. */
 int psp_hip_create(const psp_hip_desc* desc, psp_hip_ctx** out);
 
-/* ~DistHelmholtz + Finalize (initialize_finalize.hpp:379-396). */
+/* DistHelmholtz::Finalize (initialize_finalize.hpp:379-396): frees the
+ * factors and the solve workspace; the plan stays, so a later
+ * psp_hip_factor (Initialize) rebuilds them.  Solve in between is a
+ * PSP_HIP_ERR_STATE error. */
+int psp_hip_finalize(psp_hip_ctx* ctx);
+
+/* ~DistHelmholtz: frees everything, including the context. */
 int psp_hip_destroy(psp_hip_ctx* ctx);
+
+/* viewIterates (solve.hpp:579-585): with a callback set, every GMRES step
+ * that has not converged hands the current iterate x (host copy, k
+ * right-hand sides in grid layout) and the step index to fn before the next
+ * step; NULL removes it. */
+typedef void (*psp_hip_iterate_fn)(void* user, int iteration, const void* x, int k);
+int psp_hip_set_iterate_callback(psp_hip_ctx* ctx, psp_hip_iterate_fn fn, void* user);
 
 /* Message of the last failing call on this context (never NULL). */
 const char* psp_hip_last_error(const psp_hip_ctx* ctx);
@@ -228,6 +241,59 @@ int psp_hip_reordered_index(const psp_hip_ctx* ctx, int x, int y, int vLocal,
 
 /* Device inventory string (host). */
 int psp_hip_device_count(void);
+
+/* ---- Grid container I/O (host only, no device needed) -------------------
+ * DistUniformGrid<F> at one process stores numScalars values per point,
+ * points in the grid's GridDataOrder (the fastest axis first), so point
+ * (x, y, z) starts at element psp_hip_grid_index(...) * numScalars.
+ * Scalars are doubles (PSP_HIP_REAL64) or interleaved complex doubles
+ * (PSP_HIP_COMPLEX128).  Errors: PSP_HIP_ERR_INVALID, message in
+ * psp_hip_grid_last_error() (per host thread). */
+
+/* GridDataOrder (include/psp/dist_uniform_grid.hpp:15-22). */
+enum { PSP_HIP_XYZ = 0, PSP_HIP_XZY = 1, PSP_HIP_YXZ = 2, PSP_HIP_YZX = 3, PSP_HIP_ZXY = 4,
+       PSP_HIP_ZYX = 5 };
+/* PlaneType (include/psp/dist_uniform_grid.hpp:24-28). */
+enum { PSP_HIP_PLANE_XY = 0, PSP_HIP_PLANE_XZ = 1, PSP_HIP_PLANE_YZ = 2 };
+/* Scalar type of a grid. */
+enum { PSP_HIP_REAL64 = 0, PSP_HIP_COMPLEX128 = 1 };
+
+const char* psp_hip_grid_last_error(void);
+
+/* DistUniformGrid::LocalIndex / numScalars (dist_uniform_grid.hpp:302-342)
+ * at one process; -1 for an out-of-range point or order. */
+long long psp_hip_grid_index(int nx, int ny, int nz, int order, int x, int y, int z);
+
+/* DistUniformGrid::SequentialLoad (dist_uniform_grid.hpp:349-482): fills
+ * data from a raw binary file holding the grid's points in its order,
+ * numScalars native-endian scalars each.  A missing or short file is an
+ * error ("Could not open <file>"). */
+int psp_hip_grid_load(const char* filename, int nx, int ny, int nz, int numScalars, int type,
+                      int order, void* data);
+
+/* Copy between two storage orders of one grid (no reference counterpart:
+ * the C++ and Python grids use it to hand natural-order XYZ data to the
+ * solver). */
+int psp_hip_grid_reorder(const void* in, int inOrder, void* out, int outOrder, int nx, int ny,
+                         int nz, int numScalars, int type);
+
+/* DistUniformGrid::InterpolateTo (dist_uniform_grid.hpp:484-714): trilinear
+ * resampling of an nx0 x ny0 x nz0 grid onto nx x ny x nz points spanning
+ * the same box (both in `order`). */
+int psp_hip_grid_interpolate(const void* in, int nx0, int ny0, int nz0, void* out, int nx, int ny,
+                             int nz, int numScalars, int type, int order);
+
+/* DistUniformGrid::WritePlane (dist_uniform_grid.hpp:716-1629): ascii VTK
+ * ImageData files basename_<k>.vti (real) or basename_<k>_real.vti and
+ * basename_<k>_imag.vti (complex), one per scalar k. */
+int psp_hip_grid_write_plane(const void* data, int nx, int ny, int nz, int numScalars, int type,
+                             int order, int planeType, int whichPlane, const char* basename);
+
+/* DistUniformGrid::WriteVolume (dist_uniform_grid.hpp:1786-2198) at one
+ * process: basename_<k>[_real|_imag].pvti plus the piece
+ * basename_<k>[_real|_imag]/0.vti. */
+int psp_hip_grid_write_volume(const void* data, int nx, int ny, int nz, int numScalars, int type,
+                              int order, const char* basename);
 
 #ifdef __cplusplus
 }

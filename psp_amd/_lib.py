@@ -25,14 +25,21 @@ OPT_SOLVE_PERSIST_GRID = 6
 
 # Every entry point include/psp_hip.h declares.
 EXPORTS = (
-    "psp_hip_create", "psp_hip_destroy", "psp_hip_last_error", "psp_hip_set_stream",
+    "psp_hip_create", "psp_hip_finalize", "psp_hip_destroy", "psp_hip_set_iterate_callback",
+    "psp_hip_last_error", "psp_hip_set_stream",
     "psp_hip_set_option",
     "psp_hip_factor", "psp_hip_solve", "psp_hip_multiply", "psp_hip_precondition",
     "psp_hip_solve_panel", "psp_hip_solve_fixed", "psp_hip_adjoint_gradient",
     "psp_hip_profile_sweep", "psp_hip_solve_batch", "psp_hip_autotune",
     "psp_hip_num_panels", "psp_hip_panel_info", "psp_hip_class_num_nodes",
     "psp_hip_class_nodes", "psp_hip_reordered_index", "psp_hip_device_count",
+    "psp_hip_grid_last_error", "psp_hip_grid_index", "psp_hip_grid_load",
+    "psp_hip_grid_reorder", "psp_hip_grid_interpolate", "psp_hip_grid_write_plane",
+    "psp_hip_grid_write_volume",
 )
+XYZ, XZY, YXZ, YZX, ZXY, ZYX = range(6)
+PLANE_XY, PLANE_XZ, PLANE_YZ = range(3)
+REAL64, COMPLEX128 = 0, 1
 
 
 class Desc(C.Structure):
@@ -79,6 +86,9 @@ class SolveReport(C.Structure):
 
 _LIB = None
 
+# void (*)(void* user, int iteration, const void* x, int k)
+ITERATE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_void_p, C.c_int)
+
 
 def lib():
     """Load libpsp_hip.so and declare every C ABI signature."""
@@ -95,6 +105,8 @@ def lib():
     sig = {
         "psp_hip_create": (I, [C.POINTER(Desc), C.POINTER(P)]),
         "psp_hip_destroy": (I, [P]),
+        "psp_hip_finalize": (I, [P]),
+        "psp_hip_set_iterate_callback": (I, [P, ITERATE_FN, P]),
         "psp_hip_last_error": (C.c_char_p, [P]),
         "psp_hip_set_stream": (I, [P, P]),
         "psp_hip_set_option": (I, [P, I, I]),
@@ -114,6 +126,13 @@ def lib():
         "psp_hip_class_nodes": (I, [P, I, P, P, P, P, P]),
         "psp_hip_reordered_index": (I, [P, I, I, I, I]),
         "psp_hip_device_count": (I, []),
+        "psp_hip_grid_last_error": (C.c_char_p, []),
+        "psp_hip_grid_index": (C.c_longlong, [I, I, I, I, I, I, I]),
+        "psp_hip_grid_load": (I, [C.c_char_p, I, I, I, I, I, I, P]),
+        "psp_hip_grid_reorder": (I, [P, I, P, I, I, I, I, I, I]),
+        "psp_hip_grid_interpolate": (I, [P, I, I, I, P, I, I, I, I, I, I]),
+        "psp_hip_grid_write_plane": (I, [P, I, I, I, I, I, I, I, I, C.c_char_p]),
+        "psp_hip_grid_write_volume": (I, [P, I, I, I, I, I, I, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

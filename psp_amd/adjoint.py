@@ -46,7 +46,7 @@ class HelmholtzMisfit(torch.autograd.Function):
         vel = (1.0 / torch.sqrt(m)).detach().cpu().numpy()
         helm.Initialize(vel)
         U = sources.detach().clone().contiguous()
-        helm.Solve(U, restart, tol, max_its)
+        helm.Solve(U, restart, tol, maxIts=max_its)
         r = U[receivers] - data
         loss = 0.5 * (r.real.square().sum() + r.imag.square().sum())
         ctx.helm, ctx.restart, ctx.tol, ctx.max_its = helm, restart, tol, max_its
@@ -60,7 +60,7 @@ class HelmholtzMisfit(torch.autograd.Function):
         helm = ctx.helm
         Lam = torch.zeros_like(U)
         Lam[receivers] = torch.conj(r)
-        helm.Solve(Lam, ctx.restart, ctx.tol, ctx.max_its)
+        helm.Solve(Lam, ctx.restart, ctx.tol, maxIts=ctx.max_its)
         grad = torch.zeros(ctx.N, dtype=torch.float64, device=U.device)
         helm.AdjointGradient(U, Lam, grad)
         ctx.helm = None

@@ -5,6 +5,7 @@
 // exactly as InternalSolveWithGMRES (solve.hpp:241-611) does it.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -52,6 +53,7 @@ struct psp_hip_ctx {
     std::vector<void*> wsAllocs;
     int64_t wsSlots = 0;  // partial-sum slots the workspace holds
     std::vector<void*> facAllocs;  // Schur-complement and inversion scratch, only while factoring
+    std::vector<void*> facStore;   // L and LT of every panel, from Initialize to Finalize
     void *Xp = nullptr, *W = nullptr;
     void* P = nullptr;  // partial sums of split solve row blocks (64 rows x k per slot)
     void* G[3] = {nullptr, nullptr, nullptr};  // per class: the children's update rows, two slot copies
@@ -68,6 +70,8 @@ struct psp_hip_ctx {
         hipGraphExec_t exec = nullptr;
     };
     std::vector<SweepGraph> graphs;
+    psp_hip_iterate_fn iterCb = nullptr;  // called with every unconverged iterate
+    void* iterUser = nullptr;
 };
 
 namespace {
@@ -255,7 +259,6 @@ int EnsureDevice(psp_hip_ctx* c)
     if ((st = Alloc(c, &vp, sizeof(double) * (size_t)d.nx * d.ny * d.nz, c->allocs))) return st;
     c->vel = (double*)vp;
 
-    const size_t eb = FrontBytes(c);
     int dev = 0, cus = 0;
     HIPCHK(c, hipGetDevice(&dev));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -320,20 +323,46 @@ int EnsureDevice(psp_hip_ctx* c)
         D.nfold = (int)P.foldBack.size();
         std::vector<c128> s3c = ToC128(s3a);
         if ((st = Upload(c, &D.s3art, s3c))) return st;
+    }
+    c->devReady = true;
+    return PSP_HIP_OK;
+}
+
+// Factor storage (L and LT of every panel), allocated by the first
+// Initialize after construction or Finalize and kept until Finalize.
+int EnsureFactorStorage(psp_hip_ctx* c)
+{
+    const size_t eb = FrontBytes(c);
+    for (int cl = 0; cl < 3; ++cl) {
+        if (!c->plan.present[cl]) continue;
+        ClassDev& D = c->dev[cl];
+        if (D.L) continue;
+        const ClassPlan& P = c->plan.cls[cl];
         // factor arrays end in kSolvePad columns of the tallest front (zeros)
         int64_t maxFs = 1;
         for (const Node& nd : P.nodes) maxFs = std::max<int64_t>(maxFs, nd.size + nd.usize);
         const size_t fb = eb * ((size_t)D.npanels * D.fstore + (size_t)kSolvePad * maxFs);
-        if ((st = Alloc(c, &D.L, fb, c->allocs))) return st;
-        if ((st = Alloc(c, &D.LT, fb, c->allocs))) return st;
+        int st;
+        if ((st = Alloc(c, &D.L, fb, c->facStore))) return st;
+        if ((st = Alloc(c, &D.LT, fb, c->facStore))) return st;
         // only the padding needs zeroing here: every factorization zeroes L
         // before assembly, and the fold writes every entry of LT
         const size_t body = eb * (size_t)D.npanels * D.fstore;
         HIPCHK(c, hipMemsetAsync((char*)D.L + body, 0, fb - body, c->stream));
         HIPCHK(c, hipMemsetAsync((char*)D.LT + body, 0, fb - body, c->stream));
     }
-    c->devReady = true;
     return PSP_HIP_OK;
+}
+
+void FreeFactorStorage(psp_hip_ctx* c)
+{
+    for (void* p : c->facStore) (void)hipFree(p);
+    c->facStore.clear();
+    for (int cl = 0; cl < 3; ++cl) {
+        c->dev[cl].L = nullptr;
+        c->dev[cl].LT = nullptr;
+    }
+    c->factored = false;
 }
 
 void FreeGraphs(psp_hip_ctx* c)
@@ -806,6 +835,11 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
                 converged = true;
                 break;
             }
+            if (c->iterCb) {  // viewIterates (solve.hpp:579-585): the unconverged iterate
+                std::vector<c128> xh((size_t)n);
+                HIPCHK(c, hipMemcpy(xh.data(), c->x, sizeof(c128) * n, hipMemcpyDeviceToHost));
+                c->iterCb(c->iterUser, o.it - 1, xh.data(), k);
+            }
             if (maxIts > 0 && o.it >= maxIts)
                 return Fail(c, PSP_HIP_ERR_NOCONV, "GMRES reached the iteration cap");
         }
@@ -869,8 +903,25 @@ namespace {
 template <typename T>
 int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
 {
+    // Candidate shapes replace the device tile lists that captured sweeps
+    // read: drop the graphs first, and on any failure put the old shapes and
+    // their tiles back.
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    FreeGraphs(c);
+    std::array<std::vector<std::array<SolveShape, 2>>, 3> saved;
+    for (int cl = 0; cl < 3; ++cl)
+        if (c->plan.present[cl]) saved[cl] = c->plan.cls[cl].shapes;
+    auto restore = [&](int code) {
+        (void)hipStreamSynchronize(c->stream);
+        for (int cl = 0; cl < 3; ++cl)
+            if (c->plan.present[cl]) c->plan.cls[cl].shapes = saved[cl];
+        const std::string msg = c->err;
+        (void)RebuildSolveTiles(c);
+        c->err = msg;
+        return code;
+    };
     int rc = EnsureWorkspace(c, k, 1);
-    if (rc) return rc;
+    if (rc) return restore(rc);
     hipStream_t s = c->stream;
     hipEvent_t ea, eb;
     HIPCHK(c, hipEventCreate(&ea));
@@ -946,8 +997,8 @@ int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
     }
     (void)hipEventDestroy(ea);
     (void)hipEventDestroy(eb);
-    if (rc) return rc;
-    if ((rc = RebuildSolveTiles(c))) return rc;
+    if (rc) return restore(rc);
+    if ((rc = RebuildSolveTiles(c))) return restore(rc);
     if (before) *before = tb;
     if (after) *after = ta;
     return PSP_HIP_OK;
@@ -979,12 +1030,31 @@ int psp_hip_create(const psp_hip_desc* desc, psp_hip_ctx** out)
     return PSP_HIP_OK;
 }
 
+int psp_hip_finalize(psp_hip_ctx* c)
+{
+    if (!c) return PSP_HIP_ERR_INVALID;
+    if (c->devReady) (void)hipStreamSynchronize(c->stream);
+    FreeWorkspace(c);
+    FreeFactorScratch(c);
+    FreeFactorStorage(c);
+    return PSP_HIP_OK;
+}
+
+int psp_hip_set_iterate_callback(psp_hip_ctx* c, psp_hip_iterate_fn fn, void* user)
+{
+    if (!c) return PSP_HIP_ERR_INVALID;
+    c->iterCb = fn;
+    c->iterUser = user;
+    return PSP_HIP_OK;
+}
+
 int psp_hip_destroy(psp_hip_ctx* c)
 {
     if (!c) return PSP_HIP_OK;
     if (c->devReady) (void)hipStreamSynchronize(c->stream);
     FreeWorkspace(c);
     FreeFactorScratch(c);
+    FreeFactorStorage(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
@@ -1057,6 +1127,10 @@ int psp_hip_factor(psp_hip_ctx* c, const double* velocity, int scheme, psp_hip_f
     const double t0 = Now();
     int st = EnsureDevice(c);
     if (st) return st;
+    if ((st = EnsureFactorStorage(c))) return st;
+    // a new factorization invalidates the captured sweeps' results only, not
+    // their launches: the graphs keep reading the same factor arrays
+    c->factored = false;
     const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
     HIPCHK(c, hipMemcpyAsync(c->vel, velocity, sizeof(double) * N, hipMemcpyHostToDevice,
                              c->stream));

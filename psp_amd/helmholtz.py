@@ -23,7 +23,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (DIRICHLET, FRONTS_FP32, FRONTS_FP64, LDL_1D, LDL_SELINV_1D,
-                   PML, FactorReport, SolveReport)
+                   PML, XYZ, XZY, YXZ, YZX, ZXY, ZYX, FactorReport, SolveReport)
+from ._lib import PLANE_XY as XY, PLANE_XZ as XZ, PLANE_YZ as YZ
 
 CLIQUE_LDL_1D = LDL_1D
 CLIQUE_LDL_SELINV_1D = LDL_SELINV_1D
@@ -97,11 +98,17 @@ class Discretization:
 class DistUniformGrid:
     """DistUniformGrid<F> at one process (dist_uniform_grid.hpp:48-145).
 
-    Data layout: buffer[(x + y*nx + z*nx*ny) * numScalars + r].
+    numScalars values per point; points stored in the grid's GridDataOrder
+    (XYZ by default: buffer[(x + y*nx + z*nx*ny) * numScalars + r]).  data
+    has shape (nx*ny*nz, numScalars), row = LocalIndex(x, y, z) / numScalars.
+    F is float64 (velocity) or complex128 (right-hand sides, solutions).
     """
 
-    def __init__(self, numScalars, nx, ny, nz, dtype=np.complex128, data=None):
+    def __init__(self, numScalars, nx, ny, nz, dtype=np.complex128, data=None, order=XYZ):
+        if order not in (XYZ, XZY, YXZ, YZX, ZXY, ZYX):
+            raise LogicError("unknown GridDataOrder")
         self.nx, self.ny, self.nz, self.numScalars = nx, ny, nz, numScalars
+        self.order = order
         if data is None:
             data = np.zeros((nx * ny * nz, numScalars), dtype=dtype)
         self.data = data
@@ -118,11 +125,87 @@ class DistUniformGrid:
     def NumScalars(self):
         return self.numScalars
 
+    def Order(self):
+        return self.order
+
     def Buffer(self):
         return self.data
 
+    def NaturalIndex(self, x, y, z):
+        return x + y * self.nx + z * self.nx * self.ny
+
     def LocalIndex(self, x, y, z):
-        return (x + y * self.nx + z * self.nx * self.ny) * self.numScalars
+        i = _lib.lib().psp_hip_grid_index(self.nx, self.ny, self.nz, self.order, x, y, z)
+        if i < 0:
+            raise LogicError("point outside the grid")
+        return i * self.numScalars
+
+    # ---- on-disk formats (dist_uniform_grid.hpp:349-2198) ----------------
+    def _type(self):
+        if self.data.dtype == np.complex128:
+            return _lib.COMPLEX128
+        if self.data.dtype == np.float64:
+            return _lib.REAL64
+        raise LogicError("grid I/O supports float64 and complex128 grids")
+
+    def _check(self, st, what):
+        if st != _lib.OK:
+            raise LogicError(f"{what}: {_lib.lib().psp_hip_grid_last_error().decode()}")
+
+    def _buf(self):
+        if not (isinstance(self.data, np.ndarray) and self.data.flags.c_contiguous):
+            raise LogicError("grid I/O needs a C-contiguous host buffer")
+        return self.data.ctypes.data_as(C.c_void_p)
+
+    def SequentialLoad(self, filename):
+        """Fill the grid from a raw binary file holding its points in the
+        grid's order, numScalars native-endian values each."""
+        self._check(_lib.lib().psp_hip_grid_load(str(filename).encode(), self.nx, self.ny,
+                                                 self.nz, self.numScalars, self._type(),
+                                                 self.order, self._buf()),
+                    "SequentialLoad")
+
+    def InterpolateTo(self, nx, ny, nz):
+        """Trilinear resampling onto nx x ny x nz points spanning the same box."""
+        if (nx, ny, nz) == (self.nx, self.ny, self.nz):
+            return
+        out = np.zeros((nx * ny * nz, self.numScalars), dtype=self.data.dtype)
+        self._check(_lib.lib().psp_hip_grid_interpolate(
+            self._buf(), self.nx, self.ny, self.nz, out.ctypes.data_as(C.c_void_p), nx, ny, nz,
+            self.numScalars, self._type(), self.order), "InterpolateTo")
+        self.nx, self.ny, self.nz, self.data = nx, ny, nz, out
+
+    def WritePlane(self, planeType, whichPlane, basename):
+        """ASCII VTK ImageData of one XY, XZ or YZ plane per scalar."""
+        self._check(_lib.lib().psp_hip_grid_write_plane(
+            self._buf(), self.nx, self.ny, self.nz, self.numScalars, self._type(), self.order,
+            planeType, whichPlane, str(basename).encode()), "WritePlane")
+
+    def WriteVolume(self, basename):
+        """ASCII VTK parallel ImageData (one piece) of the whole grid per scalar."""
+        self._check(_lib.lib().psp_hip_grid_write_volume(
+            self._buf(), self.nx, self.ny, self.nz, self.numScalars, self._type(), self.order,
+            str(basename).encode()), "WriteVolume")
+
+    def natural(self):
+        """The data in natural XYZ order (a copy unless the grid is XYZ)."""
+        if self.order == XYZ:
+            return self.data
+        out = np.empty_like(self.data)
+        self._check(_lib.lib().psp_hip_grid_reorder(
+            self._buf(), self.order, out.ctypes.data_as(C.c_void_p), XYZ, self.nx, self.ny,
+            self.nz, self.numScalars, self._type()), "reorder")
+        return out
+
+    def set_natural(self, nat):
+        """Store natural XYZ-order data in the grid's order."""
+        nat = np.ascontiguousarray(nat, dtype=self.data.dtype).reshape(self.data.shape)
+        if self.order == XYZ:
+            self.data[...] = nat
+            return
+        self._check(_lib.lib().psp_hip_grid_reorder(
+            nat.ctypes.data_as(C.c_void_p), XYZ, self._buf(), self.order, self.nx, self.ny,
+            self.nz, self.numScalars, self._type()), "reorder")
 
 
 GridData = DistUniformGrid
@@ -163,11 +246,21 @@ class DistHelmholtz:
     # ---- lifecycle -------------------------------------------------------
     def __del__(self):
         try:
-            self.Finalize()
+            self.destroy()
         except Exception:  # interpreter shutdown: the library may be gone already
             pass
 
     def Finalize(self):
+        """Free the factors and the solve workspace
+        (initialize_finalize.hpp:379-396); the object stays usable and a later
+        Initialize rebuilds them."""
+        st = _lib.lib().psp_hip_finalize(self._ctx())
+        if st != _lib.OK:
+            _raise(st, self._h, "Finalize")
+        self.factor_report = None
+
+    def destroy(self):
+        """Release everything, the context included (the C++ destructor)."""
         h = getattr(self, "_h", None)
         if h:
             _lib.lib().psp_hip_destroy(h)
@@ -175,7 +268,7 @@ class DistHelmholtz:
 
     def _ctx(self):
         if not self._h:
-            raise LogicError("DistHelmholtz was finalized")
+            raise LogicError("DistHelmholtz was destroyed")
         return self._h
 
     def _bind_stream(self):
@@ -225,12 +318,13 @@ class DistHelmholtz:
     # ---- Initialize ------------------------------------------------------
     def Initialize(self, velocity, panelScheme=CLIQUE_LDL_SELINV_1D):
         """Factor every panel (initialize_finalize.hpp:12-377)."""
-        v = velocity.data if isinstance(velocity, DistUniformGrid) else velocity
+        v = velocity
         if isinstance(velocity, DistUniformGrid):
             if (velocity.nx, velocity.ny, velocity.nz) != (self.disc.nx, self.disc.ny, self.disc.nz):
                 raise LogicError("Velocity grid is incorrect")
             if velocity.numScalars != 1:
                 raise LogicError("Velocity grid should have one entry per point")
+            v = velocity.natural()
         v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
         if v.size != self.N:
             raise LogicError("Velocity grid is incorrect")
@@ -243,9 +337,48 @@ class DistHelmholtz:
         return rep
 
     # ---- Solve -----------------------------------------------------------
-    def Solve(self, B, m=20, relTol=1e-4, maxIts=0, raise_on_error=True):
-        """Solve in place (solve.hpp:12-56); returns the SolveReport."""
-        arr = B.data if isinstance(B, DistUniformGrid) else B
+    def Solve(self, B, m=20, relTol=1e-4, viewIterates=False, *, maxIts=0,
+              raise_on_error=True):
+        """Solve in place (solve.hpp:12-56); returns the SolveReport.
+
+        B: a DistUniformGrid of complex128 (any GridDataOrder), a host
+        (N, k) complex128 array, or a torch complex128 CUDA tensor, the last
+        two in natural grid order.  viewIterates: every unconverged GMRES
+        iterate is written as the VTK volume "iterates-<it>"
+        (solve.hpp:579-585).  maxIts > 0 caps the GMRES steps (the reference
+        has no cap)."""
+        grid = B if isinstance(B, DistUniformGrid) else None
+        if grid is not None:
+            if (grid.nx, grid.ny, grid.nz) != (self.disc.nx, self.disc.ny, self.disc.nz):
+                raise LogicError("B grid is incorrect")
+            arr = np.ascontiguousarray(grid.natural())
+        else:
+            arr = B
+        cb = None
+        if viewIterates:
+            k_ = arr.shape[1] if arr.ndim == 2 else 1
+            view = grid if grid is not None else DistUniformGrid(
+                k_, self.disc.nx, self.disc.ny, self.disc.nz)
+
+            def _dump(user, it, x, k):
+                nat = np.ctypeslib.as_array(C.cast(x, C.POINTER(C.c_double)),
+                                            shape=(self.N * k * 2,)).view(np.complex128)
+                snap = DistUniformGrid(k, view.nx, view.ny, view.nz, order=view.order)
+                snap.set_natural(nat.reshape(self.N, k))
+                snap.WriteVolume(f"iterates-{it}")
+
+            cb = _lib.ITERATE_FN(_dump)
+        _lib.lib().psp_hip_set_iterate_callback(self._ctx(), cb if cb else _lib.ITERATE_FN(),
+                                                None)
+        try:
+            rep = self._solve(arr, m, relTol, maxIts, raise_on_error)
+        finally:
+            _lib.lib().psp_hip_set_iterate_callback(self._ctx(), _lib.ITERATE_FN(), None)
+        if grid is not None:
+            grid.set_natural(arr)
+        return rep
+
+    def _solve(self, arr, m, relTol, maxIts, raise_on_error):
         rep = SolveReport()
         if _is_torch_cuda(arr):
             import torch

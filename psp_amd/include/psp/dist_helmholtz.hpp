@@ -8,7 +8,9 @@
 
 #include <complex>
 #include <cstdio>
+#include <string>
 #include <type_traits>
+#include <vector>
 
 namespace psp {
 
@@ -57,7 +59,10 @@ public:
         if (st != PSP_HIP_OK) LogicError("invalid discretization (status ", st, ")");
     }
 
-    ~DistHelmholtz() { Finalize(); }
+    ~DistHelmholtz()
+    {
+        if (ctx_) psp_hip_destroy(ctx_);
+    }
     DistHelmholtz(const DistHelmholtz&) = delete;
     DistHelmholtz& operator=(const DistHelmholtz&) = delete;
 
@@ -70,18 +75,45 @@ public:
             disc_.nz != velocity.ZSize())
             LogicError("Velocity grid is incorrect");
         if (velocity.NumScalars() != 1) LogicError("Velocity grid should have one entry per point");
-        ThrowOnStatus(psp_hip_factor(ctx_, velocity.LockedBuffer(), panelScheme, &factorReport_),
-                      ctx_, "Initialize");
+        if (velocity.Order() == XYZ) {
+            ThrowOnStatus(psp_hip_factor(ctx_, velocity.LockedBuffer(), panelScheme,
+                                         &factorReport_),
+                          ctx_, "Initialize");
+        } else {
+            const std::vector<Real> nat = velocity.Natural();
+            ThrowOnStatus(psp_hip_factor(ctx_, nat.data(), panelScheme, &factorReport_), ctx_,
+                          "Initialize");
+        }
     }
 
     // Solves an O(1) set of linear systems with the sweeping preconditioner
     // (solve.hpp:12-56); B is overwritten with the solutions.
+    // With viewIterates every unconverged GMRES iterate is written as the VTK
+    // volume "iterates-<it>" (solve.hpp:579-585).
     void Solve(DistUniformGrid<C>& B, int m = 20, Real relTol = 1e-4,
                bool viewIterates = false) const
     {
-        (void)viewIterates;  // VTK iterate dumps are not written
-        const int st = psp_hip_solve(ctx_, B.Buffer(), B.NumScalars(), m, relTol, 0, 0,
-                                     &solveReport_);
+        if (B.XSize() != disc_.nx || B.YSize() != disc_.ny || B.ZSize() != disc_.nz)
+            LogicError("B grid is incorrect");
+        struct View {
+            DistUniformGrid<C>* grid;
+            static void Dump(void* user, int it, const void* x, int)
+            {
+                DistUniformGrid<C>& g = *((View*)user)->grid;
+                g.SetNatural((const C*)x);
+                g.WriteVolume("iterates-" + std::to_string(it));
+            }
+        } view{&B};
+        psp_hip_set_iterate_callback(ctx_, viewIterates ? &View::Dump : nullptr, &view);
+        int st;
+        if (B.Order() == XYZ) {
+            st = psp_hip_solve(ctx_, B.Buffer(), B.NumScalars(), m, relTol, 0, 0, &solveReport_);
+        } else {
+            std::vector<C> nat = B.Natural();
+            st = psp_hip_solve(ctx_, nat.data(), B.NumScalars(), m, relTol, 0, 0, &solveReport_);
+            B.SetNatural(nat.data());
+        }
+        psp_hip_set_iterate_callback(ctx_, nullptr, nullptr);
         ThrowOnStatus(st, ctx_, "Solve");
     }
 
@@ -92,12 +124,9 @@ public:
     }
 
     // Destroy the sparse matrix and the preconditioner
-    // (initialize_finalize.hpp:379-396).
-    void Finalize()
-    {
-        if (ctx_) psp_hip_destroy(ctx_);
-        ctx_ = nullptr;
-    }
+    // (initialize_finalize.hpp:379-396); the object stays usable and a later
+    // Initialize rebuilds them.
+    void Finalize() { ThrowOnStatus(psp_hip_finalize(ctx_), ctx_, "Finalize"); }
 
     const psp_hip_factor_report& FactorReport() const { return factorReport_; }
     const psp_hip_solve_report& SolveReport() const { return solveReport_; }

@@ -84,39 +84,58 @@ def interior_points(nx, ny, nz, count, cfg, pml=5):
     return pts
 
 
-def thrust_belt_velocity(nx, ny, nz, cfg, nlayers=12, nfaults=3):
-    """Layered thrust-belt model: velocity rising 2.0 -> 6.0 with depth (z)
-    through seeded dipping interfaces, offset by planar thrust faults, with
-    +-5% smooth noise."""
+def thrust_belt_model(nx, ny, nz, cfg, nlayers=12, nfaults=3):
+    """Seeded parameters of the layered thrust-belt model, lengths in grid
+    cells: faults (strike, offset, throw) and interfaces (depth z0 at the
+    model centre, dip angle, dip direction), both drawn from PCG64 stream
+    (cfg, 1).  Dips are U[0, 30] degrees, throws U[5, 15] cells."""
     rng = _rng(cfg, 1)
-    xx, yy, zz, h = grid_coords(nx, ny, nz)
-    depth = zz / (nz * h)  # 0 at top, ~1 at bottom
-    X = xx / (nx * h)
-    Y = yy / (ny * h)
-    shift = np.zeros_like(depth)
+    faults = []
     for _ in range(nfaults):
         strike = rng.uniform(0, np.pi)
-        nrm = np.array([np.cos(strike), np.sin(strike)])
         off = rng.uniform(0.3, 0.7)
-        throw = rng.uniform(5, 15) / nz
-        side = (X * nrm[0] + Y * nrm[1] - off + 0.3 * depth) > 0
-        shift = shift + np.where(side, throw, 0.0)
-    d = depth - shift
-    ifaces = np.sort(rng.uniform(0.05, 0.95, nlayers))
-    layer = np.zeros_like(d)
-    for i, z0 in enumerate(ifaces):
+        throw = rng.uniform(5, 15)
+        faults.append((strike, off, throw))
+    z0s = np.sort(rng.uniform(0.05, 0.95, nlayers)) * nz
+    ifaces = []
+    for z0 in z0s:
         dip = np.deg2rad(rng.uniform(0, 30))
-        strike = rng.uniform(0, 2 * np.pi)
-        plane = z0 + np.tan(dip) * 0.1 * ((X - 0.5) * np.cos(strike) + (Y - 0.5) * np.sin(strike))
+        azim = rng.uniform(0, 2 * np.pi)
+        ifaces.append((float(z0), float(dip), float(azim)))
+    return dict(faults=faults, interfaces=ifaces, rng=rng)
+
+
+def thrust_belt_velocity(nx, ny, nz, cfg, nlayers=12, nfaults=3, noise_amp=0.05):
+    """Layered thrust-belt model: velocity rising 2.0 -> 6.0 with depth (z)
+    through seeded dipping interfaces, offset by planar thrust faults, with
+    +-noise_amp smooth noise.  Interfaces are planes
+    z = z0 + tan(dip) * (horizontal distance from the model centre along the
+    dip direction), all lengths in grid cells, so a drawn dip is the physical
+    dip of the interface."""
+    mdl = thrust_belt_model(nx, ny, nz, cfg, nlayers, nfaults)
+    rng = mdl["rng"]
+    xx, yy, zz, h = grid_coords(nx, ny, nz)
+    gz, gy, gx = zz / h, yy / h, xx / h
+    X = gx / nx
+    Y = gy / ny
+    shift = np.zeros_like(gz)
+    for strike, off, throw in mdl["faults"]:
+        side = (X * np.cos(strike) + Y * np.sin(strike) - off + 0.3 * gz / nz) > 0
+        shift = shift + np.where(side, throw, 0.0)
+    d = gz - shift  # depth in cells in the unfaulted frame
+    layer = np.zeros_like(d)
+    for z0, dip, azim in mdl["interfaces"]:
+        plane = z0 + np.tan(dip) * ((gx - (nx - 1) / 2) * np.cos(azim)
+                                    + (gy - (ny - 1) / 2) * np.sin(azim))
         layer = layer + (d > plane)
-    c = 2.0 + 4.0 * layer / nlayers
+    c = 2.0 + 4.0 * layer / max(nlayers, 1)
     noise = np.zeros_like(c)
     for _ in range(16):
         cx, cy, cz = rng.uniform(0, 1, 3) * np.array([nx, ny, nz]) * h
         w = rng.uniform(8, 30) * h
         noise += rng.uniform(-1, 1) * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2) / (2 * w * w))
     noise /= max(1e-12, np.abs(noise).max())
-    return (c * (1 + 0.05 * noise)).reshape(-1)
+    return (c * (1 + noise_amp * noise)).reshape(-1)
 
 
 @dataclass

@@ -136,3 +136,21 @@ def test_solve_tiling_options_checked_on_host():
         h.set_solve_tiling(max_split=0)
     with pytest.raises(psp_amd.LogicError, match="min tiles"):
         h.set_solve_tiling(min_tiles=0)
+
+
+def test_finalize_keeps_the_object_usable_on_host():
+    """Finalize frees the factors only (initialize_finalize.hpp:379-396): the
+    object stays valid, Solve afterwards is a LogicError (not initialized),
+    and Finalize may be called again; destroy releases the context."""
+    import psp_amd
+    h = psp_amd.DistHelmholtz(psp_amd.unit_cube_discretization(10, nz=30))
+    panels = h.num_panels
+    h.Finalize()
+    h.Finalize()
+    assert h.num_panels == panels
+    B = np.zeros((10 * 10 * 30, 1), dtype=np.complex128)
+    with pytest.raises(psp_amd.LogicError, match="before Initialize"):
+        h.Solve(B)
+    h.destroy()
+    with pytest.raises(psp_amd.LogicError, match="destroyed"):
+        h.Finalize()

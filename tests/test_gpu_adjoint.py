@@ -80,7 +80,7 @@ def _survey_data(torch, wl, tol=1e-11):
     ht = psp_amd.DistHelmholtz(wl.disc)
     ht.Initialize(1.0 / np.sqrt(wl.m_true))
     Ut = torch.from_numpy(wl.B).cuda()
-    ht.Solve(Ut, 20, tol, 1000)
+    ht.Solve(Ut, 20, tol, maxIts=1000)
     rec = torch.from_numpy(wl.receivers).cuda()
     return Ut[rec].clone()
 
