@@ -29,16 +29,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = "k_mvp_k_mvm_front_solves_bumps100_8rhs.md"
+
+
 def measured_traffic():
     """HBM bytes per front-solve product launch (FETCH_SIZE + WRITE_SIZE,
     rocprofv3 --pmc) recorded in the committed profile summary, or None."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_mvp_k_mvm_front_solves_bumps100_8rhs.md")
+    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILE)
     try:
         m = re.search(r"\*\*([0-9.]+) MB per launch\*\* of HBM traffic", open(path).read())
         return float(m.group(1)) * 1e6 if m else None
     except OSError:
         return None
+
+
+def cpu_model():
+    """Model name of the host CPU (Linux), or None."""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(wl, out, args):
@@ -84,6 +98,7 @@ def cpu_baseline(wl, out, args):
     log(f"[cpu] sweep {t_sweep:.2f}s, multiply {t_mv:.3f}s, arnoldi cycle {t_arn:.2f}s -> "
         f"solve {t_solve:.1f}s, factor {n_fac:.1f}s (est.)")
     return {"value": k / t_solve, "unit": "RHS/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
             "factor_s_est": n_fac, "solve_s_est": t_solve,
             "sample": (f"oracle (FP64 CPU restatement, not the PSP binary): one panel of each of "
                        f"{len(rep_panel)} panel classes factored and solved once with the "
@@ -104,6 +119,7 @@ def cpu_full(wl, args):
     t_sol = time.perf_counter() - t0
     log(f"[cpu] factor {t_fac:.2f}s, solve {t_sol:.2f}s, {rep.iterations} its")
     return {"value": B.shape[1] / t_sol, "unit": "RHS/s", "cores": num_threads(), "kind": "port",
+            "cpu_model": cpu_model(), "iterations": rep.iterations,
             "factor_s": t_fac, "solve_s": t_sol,
             "sample": "oracle (FP64 CPU restatement, not the PSP binary): the whole workload"}
 
@@ -151,12 +167,14 @@ def bench_batch(args, dev):
             reps += rp
             del helms, srcs
             torch.cuda.empty_cache()
-        res[prec] = dict(factor_all_s=t_fac, solve_s=t_sol, rhs_per_s=ktot / t_sol,
+        conv = all(r.status == 0 and r.final_max_rel < args.tol for r in reps)
+        res[prec] = dict(factor_all_s=t_fac, solve_s=t_sol,
+                         rhs_per_s=ktot / t_sol if conv else None, converged=conv,
                          problems_per_batch=group,
                          max_final_rel=max(r.final_max_rel for r in reps),
                          iterations=[r.iterations for r in reps])
         log(f"[bench] batch32 {prec} ({group} problems per SolveBatch): factor all {t_fac:.2f}s, "
-            f"solve all {t_sol:.3f}s -> {ktot / t_sol:.1f} RHS/s, worst final rel "
+            f"solve all {t_sol:.3f}s -> {ktot / t_sol:.1f} RHS/s (converged {conv}), worst final rel "
             f"{res[prec]['max_final_rel']:.2e}")
     line = {"metric": "aggregate RHS solves/s over 32 uneven problems to 1e-5 rel. residual",
             "value": res["fp32"]["rhs_per_s"], "unit": "RHS/s", "n_gpus": 1,
@@ -239,6 +257,11 @@ def main():
                          "unitcube40: 40^3 c=1, 1 source; thrustbelt: 200x200x100, 64 sources; "
                          "batch32: 32 uneven problems solved together; adjoint80: 80^3 "
                          "misfit gradient, 16 sources")
+    ap.add_argument("--wide-k", type=int, default=64,
+                    help="also time FP32 and FP64 fronts with this many sources on the "
+                         "100^3 grid (0: skip)")
+    ap.add_argument("--wide-steps", type=int, default=2,
+                    help="timed solves of the wide block per precision")
     ap.add_argument("--calibrate", action="store_true",
                     help="copy a 1 GiB device buffer once (a known-size kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -283,29 +306,34 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    def run_path(precision, steps, warmup):
+    def run_path(precision, steps, warmup, src=None):
+        """Initialize + Autotune, then `warmup` untimed and `steps` timed
+        Solves of the block; returns the object, its factor report, the
+        factor time, the timed seconds, the reports and one solution."""
+        src = Bsrc if src is None else src
+        kk = src.shape[1]
         h = psp_amd.DistHelmholtz(disc, precision=precision)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rep = h.Initialize(vel)
         t_fac = time.perf_counter() - t0
-        log(f"[bench] {precision}: factor {t_fac:.3f}s (fill {rep.seconds_fill:.3f} "
+        log(f"[bench] {precision} k={kk}: factor {t_fac:.3f}s (fill {rep.seconds_fill:.3f} "
             f"ldl {rep.seconds_ldl:.3f} inv {rep.seconds_inverse:.3f}), "
             f"{rep.flops / 1e12:.2f} TFLOP, {rep.bytes_factor / 1e9:.2f} GB of factors")
         tuned = None
         if not args.no_autotune:
             t0 = time.perf_counter()
-            before, after = h.Autotune(k)
+            before, after = h.Autotune(kk)
             tuned = dict(seconds=time.perf_counter() - t0, est_products_before_s=before,
                          est_products_after_s=after)
-            log(f"[bench] {precision}: autotuned solve tiles in {tuned['seconds']:.2f}s "
+            log(f"[bench] {precision} k={kk}: autotuned solve tiles in {tuned['seconds']:.2f}s "
                 f"(est. front-solve time per sweep {before * 1e3:.2f} -> {after * 1e3:.2f} ms)")
-        bufs = [Bsrc.clone() for _ in range(max(steps, 1))]
+        bufs = [src.clone() for _ in range(max(steps, 1))]
         for w in range(warmup):
-            x = Bsrc.clone()
+            x = src.clone()
             r = h.Solve(x, args.m, args.tol, maxIts=args.max_its, raise_on_error=False)
-            log(f"[bench] {precision} warmup {w}: {r.iterations} its, rel {r.final_max_rel:.2e}, "
-                f"{r.seconds_total:.3f}s (precond {r.seconds_precond:.3f}, "
+            log(f"[bench] {precision} k={kk} warmup {w}: {r.iterations} its, rel "
+                f"{r.final_max_rel:.2e}, {r.seconds_total:.3f}s (precond {r.seconds_precond:.3f}, "
                 f"multiply {r.seconds_multiply:.3f}, arnoldi {r.seconds_arnoldi:.3f})")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -315,18 +343,27 @@ def main():
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
         for s, r in enumerate(reps):
-            log(f"[bench] {precision} step {s}: {r.iterations} its, rel {r.final_max_rel:.2e}, "
-                f"{r.seconds_total:.3f}s" + ("" if r.status == 0 else f" (status {r.status})"))
+            log(f"[bench] {precision} k={kk} step {s}: {r.iterations} its, rel "
+                f"{r.final_max_rel:.2e}, {r.seconds_total:.3f}s"
+                + ("" if r.status == 0 else f" (status {r.status})"))
         x0 = bufs[0]
         del bufs
         rep.tuned = tuned
         return h, rep, t_fac, t, reps, x0
 
+    def summary(reps, t, t_fac, kk, f):
+        conv = all(r.status == 0 and r.final_max_rel < args.tol for r in reps)
+        out_ = dict(factor_s=t_fac, solve_s=t / len(reps), iterations=reps[0].iterations,
+                    converged=conv, rhs_per_s=len(reps) * kk / t if conv else None,
+                    final_rel=max(r.final_max_rel for r in reps), autotune=f.tuned,
+                    schur_gemm_tflops=f.schur_flops / max(f.seconds_schur, 1e-12) / 1e12)
+        if not conv:
+            out_["rhs_per_s_unconverged"] = len(reps) * kk / t
+        return out_
+
     h32, f32, tf32, t32, reps32, x32 = run_path("fp32", args.steps, args.warmup)
-    value = args.steps * k / t32
-    out["fp32"] = dict(factor_s=tf32, solve_s=t32 / args.steps, iterations=reps32[0].iterations,
-                       converged=reps32[0].status == 0,
-                       rhs_per_s=value, final_rel=reps32[0].final_max_rel, autotune=f32.tuned)
+    out["fp32"] = summary(reps32, t32, tf32, k, f32)
+    value = out["fp32"]["rhs_per_s"]
 
     # ---- roofline of the dominant kernels: the front-solve products -----
     # One more sweep of the same block, launched kernel by kernel with HIP
@@ -335,14 +372,18 @@ def main():
     launches, ksec, kbytes, kflops = h32.ProfileSweep(Bsrc.clone())
     achieved = kbytes / ksec / 1e9
     kname = ("k_mvp/k_mvm" if k <= 16 else "k_mvm") if 8 <= k < 32 else ("k_mmw" if k >= 32 else "k_mv")
+    traffic = measured_traffic() if (k == 8 and args.workload == "bumps100") else None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                 "frac": achieved / 8000.0,
-                "traffic": measured_traffic() if (k == 8 and args.workload == "bumps100") else None,
+                "traffic": traffic,
+                "traffic_source": (None if traffic is None else
+                                   "profiles/" + TRAFFIC_PROFILE + " (rocprofv3 --pmc FETCH_SIZE "
+                                   "+ WRITE_SIZE per launch, calibrated; not measured in this run)"),
                 "kernel": f"{kname} (front-solve products, c64 fronts)",
                 "launches_per_sweep": launches, "avg_launch_us": 1e6 * ksec / launches,
                 "algorithmic_bytes_per_launch": kbytes / launches,
                 "achieved_tflops": kflops / ksec / 1e12,
-                "schur_gemm_tflops": f32.schur_flops / max(f32.seconds_schur, 1e-12) / 1e12}
+                "schur_gemm_tflops": out["fp32"]["schur_gemm_tflops"]}
     log(f"[bench] roofline: {launches} front-solve launches/sweep, avg "
         f"{1e6 * ksec / launches:.1f} us, {kbytes / launches / 1e6:.2f} MB/launch -> "
         f"{achieved:.0f} GB/s, {kflops / ksec / 1e12:.2f} TFLOP/s; Schur GEMMs "
@@ -351,18 +392,40 @@ def main():
     torch.cuda.empty_cache()
 
     if not args.no_fp64:
-        h64, f64, tf64, t64, reps64, x64 = run_path("fp64", 1, 0)
+        # the FP64-front path with the same warm-ups and timed steps
+        h64, f64, tf64, t64, reps64, x64 = run_path("fp64", args.steps, args.warmup)
         err = (torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64)).item()
-        out["fp64"] = dict(factor_s=tf64, solve_s=t64, iterations=reps64[0].iterations,
-                           converged=reps64[0].status == 0,
-                           rhs_per_s=k / t64, final_rel=reps64[0].final_max_rel,
-                           autotune=f64.tuned,
-                           schur_gemm_tflops=f64.schur_flops / max(f64.seconds_schur, 1e-12) / 1e12)
+        out["fp64"] = summary(reps64, t64, tf64, k, f64)
         out["fp32_vs_fp64_rel_err"] = err
-        out["fp32_over_fp64_rhs_per_s"] = value / (k / t64)
+        if value is not None and out["fp64"]["rhs_per_s"]:
+            out["fp32_over_fp64_rhs_per_s"] = value / out["fp64"]["rhs_per_s"]
         log(f"[bench] fp32 vs fp64 solution rel err {err:.3e}")
         del h64
         torch.cuda.empty_cache()
+
+    if args.wide_k > 0:
+        # the many-sources pair: the same grid with a wide block of sources
+        wl_w = workloads.config2(n=args.n, k=args.wide_k) if args.workload == "bumps100" else None
+        if wl_w is not None:
+            Bw = torch.from_numpy(wl_w.B).to(dev)
+            wide = {}
+            xs = {}
+            for prec in ("fp32", "fp64"):
+                if prec == "fp64" and args.no_fp64:
+                    continue
+                hw, fw, tfw, tw, repsw, xw = run_path(prec, args.wide_steps, 1, Bw)
+                wide[prec] = summary(repsw, tw, tfw, args.wide_k, fw)
+                xs[prec] = xw
+                del hw
+                torch.cuda.empty_cache()
+            if "fp64" in xs:
+                wide["fp32_vs_fp64_rel_err"] = (torch.linalg.norm(xs["fp32"] - xs["fp64"]) /
+                                                torch.linalg.norm(xs["fp64"])).item()
+                if wide["fp32"]["rhs_per_s"] and wide["fp64"]["rhs_per_s"]:
+                    wide["fp32_over_fp64_rhs_per_s"] = (wide["fp32"]["rhs_per_s"] /
+                                                        wide["fp64"]["rhs_per_s"])
+            out[f"k{args.wide_k}"] = wide
+            del xs, Bw
 
     if args.no_cpu:
         cpu = None
@@ -388,7 +451,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": [disc.nx, disc.ny, disc.nz], "rhs": k,
                    "gmres_m": args.m, "rel_tol": args.tol, "omega": disc.omega,
-                   "max_its": args.max_its},
+                   "sigma": disc.sigmax, "max_its": args.max_its},
+        "status": "ok" if value is not None else
+                  "not converged: value is null (a timed solve ended above rel_tol)",
         "roofline": roofline,
         "cpu_baseline": cpu,
         "paths": out,

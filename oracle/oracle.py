@@ -43,47 +43,54 @@ class Report(C.Structure):
 
 
 def lib():
+    """The oracle library built by oracle/Makefile (loaded once)."""
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "libpsp_oracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
-        L = C.CDLL(path)
-        P, I, D, V = C.c_void_p, C.c_int, C.c_double, None
-        sig = {
-            "ora_create": (I, [C.POINTER(Disc), C.POINTER(P)]),
-            "ora_destroy": (V, [P]),
-            "ora_num_panels": (I, [P]),
-            "ora_panel_info": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
-            "ora_global_size": (C.c_long, [P]),
-            "ora_local_to_natural": (V, [P, P]),
-            "ora_reordered_index": (I, [P, I, I, I, I]),
-            "ora_class_vsize": (I, [P, I]),
-            "ora_class_num_nodes": (I, [P, I]),
-            "ora_class_nodes": (V, [P, I, P, P, P, P]),
-            "ora_class_structs": (V, [P, I, P]),
-            "ora_global_nnz": (C.c_long, [P]),
-            "ora_global_csr": (I, [P, P, P, P]),
-            "ora_panel_lower_coo": (C.c_long, [P, I, P, P, P, P]),
-            "ora_factor": (I, [P, P]),
-            "ora_factor_seconds": (D, [P]),
-            "ora_factor_sample": (I, [P, P, I, I, C.POINTER(D)]),
-            "ora_multiply": (I, [P, P, I]),
-            "ora_precondition": (I, [P, P, I]),
-            "ora_solve_panel": (I, [P, P, I, I]),
-            "ora_solve": (I, [P, P, I, I, D, I, C.POINTER(Report)]),
-            "ora_solve_fixed": (I, [P, P, I, I, I, P]),
-            "ora_partial_ldl": (I, [P, I, I]),
-            "ora_fill_global": (I, [P, P]),
-            "ora_num_threads": (I, []),
-            "ora_time_arnoldi": (D, [P, I, I]),
-        }
-        for name, (res, args) in sig.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        _LIB = L
+        _LIB = load_library(path)
     return _LIB
+
+
+def load_library(path):
+    """Loads an oracle build from `path` with its function signatures (the
+    pinning tests load builds with a planted mistake this way)."""
+    L = C.CDLL(path)
+    P, I, D, V = C.c_void_p, C.c_int, C.c_double, None
+    sig = {
+        "ora_create": (I, [C.POINTER(Disc), C.POINTER(P)]),
+        "ora_destroy": (V, [P]),
+        "ora_num_panels": (I, [P]),
+        "ora_panel_info": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+        "ora_global_size": (C.c_long, [P]),
+        "ora_local_to_natural": (V, [P, P]),
+        "ora_reordered_index": (I, [P, I, I, I, I]),
+        "ora_class_vsize": (I, [P, I]),
+        "ora_class_num_nodes": (I, [P, I]),
+        "ora_class_nodes": (V, [P, I, P, P, P, P]),
+        "ora_class_structs": (V, [P, I, P]),
+        "ora_global_nnz": (C.c_long, [P]),
+        "ora_global_csr": (I, [P, P, P, P]),
+        "ora_panel_lower_coo": (C.c_long, [P, I, P, P, P, P]),
+        "ora_factor": (I, [P, P]),
+        "ora_factor_seconds": (D, [P]),
+        "ora_factor_sample": (I, [P, P, I, I, C.POINTER(D)]),
+        "ora_multiply": (I, [P, P, I]),
+        "ora_precondition": (I, [P, P, I]),
+        "ora_solve_panel": (I, [P, P, I, I]),
+        "ora_solve": (I, [P, P, I, I, D, I, C.POINTER(Report)]),
+        "ora_solve_fixed": (I, [P, P, I, I, I, P]),
+        "ora_partial_ldl": (I, [P, I, I]),
+        "ora_fill_global": (I, [P, P]),
+        "ora_num_threads": (I, []),
+        "ora_time_arnoldi": (D, [P, I, I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
 
 
 def _ptr(a):
@@ -101,7 +108,7 @@ class Oracle:
 
     def __init__(self, omega, nx, ny, nz, wx=None, wy=None, wz=None,
                  bcs=(PML,) * 5, pml=5, sigma=1.5, damping=7.0,
-                 planes_per_panel=4, cutoff=12, sigmas=None, pmls=None):
+                 planes_per_panel=4, cutoff=12, sigmas=None, pmls=None, library=None):
         h = None
         if wx is None:
             h = 1.0 / (max(nx, ny, nz) + 1)
@@ -112,44 +119,45 @@ class Oracle:
                          sx, sy, sz, damping, planes_per_panel, cutoff)
         self.nx, self.ny, self.nz = nx, ny, nz
         self.N = nx * ny * nz
+        self._L = library if library is not None else lib()
         h_ = C.c_void_p()
-        st = lib().ora_create(C.byref(self.disc), C.byref(h_))
+        st = self._L.ora_create(C.byref(self.disc), C.byref(h_))
         if st != 0:
             raise OracleError(st, "create")
         self._h = h_
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().ora_destroy(self._h)
+            self._L.ora_destroy(self._h)
             self._h = None
 
     # geometry -----------------------------------------------------------
     @property
     def num_panels(self):
-        return lib().ora_num_panels(self._h)
+        return self._L.ora_num_panels(self._h)
 
     def panel_info(self, p):
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        lib().ora_panel_info(self._h, p, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        self._L.ora_panel_info(self._h, p, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
         return dict(v=a.value, depth=b.value, padding=c.value, cls=d.value)
 
     def local_to_natural(self):
         m = np.empty(self.N, dtype=np.int32)
-        lib().ora_local_to_natural(self._h, _ptr(m))
+        self._L.ora_local_to_natural(self._h, _ptr(m))
         return m
 
     def reordered_index(self, x, y, vl, vs):
-        return lib().ora_reordered_index(self._h, x, y, vl, vs)
+        return self._L.ora_reordered_index(self._h, x, y, vl, vs)
 
     def class_vsize(self, cls):
-        return lib().ora_class_vsize(self._h, cls)
+        return self._L.ora_class_vsize(self._h, cls)
 
     def class_nodes(self, cls):
-        n = lib().ora_class_num_nodes(self._h, cls)
+        n = self._L.ora_class_num_nodes(self._h, cls)
         off, size, parent, usize = (np.empty(n, np.int32) for _ in range(4))
-        lib().ora_class_nodes(self._h, cls, _ptr(off), _ptr(size), _ptr(parent), _ptr(usize))
+        self._L.ora_class_nodes(self._h, cls, _ptr(off), _ptr(size), _ptr(parent), _ptr(usize))
         st = np.empty(int(usize.sum()), np.int32)
-        lib().ora_class_structs(self._h, cls, _ptr(st))
+        self._L.ora_class_structs(self._h, cls, _ptr(st))
         starts = np.concatenate([[0], np.cumsum(usize)])
         structs = [st[starts[i]:starts[i + 1]] for i in range(n)]
         return dict(off=off, size=size, parent=parent, usize=usize, structs=structs)
@@ -159,19 +167,19 @@ class Oracle:
         v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
         assert v.size == self.N
         self._vel = v
-        st = lib().ora_factor(self._h, _ptr(v))
+        st = self._L.ora_factor(self._h, _ptr(v))
         if st != 0:
             raise OracleError(st, "factor")
 
     @property
     def factor_seconds(self):
-        return lib().ora_factor_seconds(self._h)
+        return self._L.ora_factor_seconds(self._h)
 
     def factor_sample(self, velocity, first, count):
         v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
         self._vel = v
         t = C.c_double()
-        st = lib().ora_factor_sample(self._h, _ptr(v), first, count, C.byref(t))
+        st = self._L.ora_factor_sample(self._h, _ptr(v), first, count, C.byref(t))
         if st != 0:
             raise OracleError(st, "factor_sample")
         return t.value
@@ -179,26 +187,26 @@ class Oracle:
     def fill_global(self, velocity):
         v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
         self._vel = v
-        lib().ora_fill_global(self._h, _ptr(v))
+        self._L.ora_fill_global(self._h, _ptr(v))
 
     def time_arnoldi(self, k, m):
-        return lib().ora_time_arnoldi(self._h, k, m)
+        return self._L.ora_time_arnoldi(self._h, k, m)
 
     def global_csr(self):
-        nnz = lib().ora_global_nnz(self._h)
+        nnz = self._L.ora_global_nnz(self._h)
         rp = np.empty(self.N + 1, np.int32)
         cols = np.empty(nnz, np.int32)
         vals = np.empty(nnz, np.complex128)
-        lib().ora_global_csr(self._h, _ptr(rp), _ptr(cols), _ptr(vals))
+        self._L.ora_global_csr(self._h, _ptr(rp), _ptr(cols), _ptr(vals))
         return rp, cols, vals
 
     def panel_lower_coo(self, panel, velocity):
         v = np.ascontiguousarray(velocity, dtype=np.float64).reshape(-1)
-        n = lib().ora_panel_lower_coo(self._h, panel, _ptr(v), None, None, None)
+        n = self._L.ora_panel_lower_coo(self._h, panel, _ptr(v), None, None, None)
         rows = np.empty(n, np.int32)
         cols = np.empty(n, np.int32)
         vals = np.empty(n, np.complex128)
-        lib().ora_panel_lower_coo(self._h, panel, _ptr(v), _ptr(rows), _ptr(cols), _ptr(vals))
+        self._L.ora_panel_lower_coo(self._h, panel, _ptr(v), _ptr(rows), _ptr(cols), _ptr(vals))
         return rows, cols, vals
 
     def _nd_block(self, B):
@@ -210,21 +218,21 @@ class Oracle:
 
     def multiply(self, B):
         B = self._nd_block(B).copy(order="F")
-        st = lib().ora_multiply(self._h, _ptr(B), B.shape[1])
+        st = self._L.ora_multiply(self._h, _ptr(B), B.shape[1])
         if st != 0:
             raise OracleError(st, "multiply")
         return B
 
     def precondition(self, B):
         B = self._nd_block(B).copy(order="F")
-        st = lib().ora_precondition(self._h, _ptr(B), B.shape[1])
+        st = self._L.ora_precondition(self._h, _ptr(B), B.shape[1])
         if st != 0:
             raise OracleError(st, "precondition")
         return B
 
     def solve_panel(self, B, panel):
         B = self._nd_block(B).copy(order="F")
-        st = lib().ora_solve_panel(self._h, _ptr(B), B.shape[1], panel)
+        st = self._L.ora_solve_panel(self._h, _ptr(B), B.shape[1], panel)
         if st != 0:
             raise OracleError(st, "solve_panel")
         return B
@@ -236,7 +244,7 @@ class Oracle:
             B = B.reshape(-1, 1)
         k = B.shape[1]
         rep = Report()
-        st = lib().ora_solve(self._h, _ptr(B), k, m, rel_tol, max_its, C.byref(rep))
+        st = self._L.ora_solve(self._h, _ptr(B), k, m, rel_tol, max_its, C.byref(rep))
         return B, st, rep
 
     def solve_fixed(self, Bgrid, m, steps):
@@ -244,7 +252,7 @@ class Oracle:
         if B.ndim == 1:
             B = B.reshape(-1, 1)
         hist = np.zeros(steps)
-        st = lib().ora_solve_fixed(self._h, _ptr(B), B.shape[1], m, steps, _ptr(hist))
+        st = self._L.ora_solve_fixed(self._h, _ptr(B), B.shape[1], m, steps, _ptr(hist))
         if st != 0:
             raise OracleError(st, "solve_fixed")
         return B, hist

@@ -351,75 +351,70 @@ static void analyse_class(const ora_ctx* X, class_t* c, int vsize)
 /* ------------------------------------------------------------------ */
 /* PML profiles: initialize_finalize.hpp:415-525                       */
 /* ------------------------------------------------------------------ */
+/* Planted mistakes for the pinning tests' mutant builds (-DORA_PLANT=n,
+ * compiled into a separate library by tests/test_oracle_pml.py); the
+ * oracle proper is built with ORA_PLANT = 0 and none of them.
+ *   1: back face of the x profile shifted by one cell
+ *   2: imaginary part of every profile negated
+ *   3: profile linear instead of quadratic in the depth
+ *   4: artificial PML of a panel one plane too deep
+ *   5: damping applied as omega - i*damping                               */
+#ifndef ORA_PLANT
+#define ORA_PLANT 0
+#endif
+
+/* 1 + i (sigma/eta) (delta/b)^2 (2 pi/omega), eta = b h: the profile value
+ * at depth delta cells into a layer of b cells (initialize_finalize.hpp:
+ * 420-426 and the seven copies after it). */
+static zc pml_value(double sigma, int b, double h, double omega, double delta)
+{
+    const double eta = b * h;
+    const double q = (sigma / eta) * (delta / b);
+    const double im = (ORA_PLANT == 3 ? q : q * (delta / b)) * (2 * M_PI / omega);
+    return 1.0 + (ORA_PLANT == 2 ? -im : im) * I;
+}
+
 static zc s1inv(const ora_ctx* X, int x)
 {
     const int bx = X->d.bx;
-    const double etax = bx * X->hx;
-    if (x < (bx - 1) && X->d.frontBC == ORA_PML) {
-        const double delta = (bx - 1) - x;
-        const double im = (X->d.sigmax / etax) * (delta / bx) * (delta / bx) *
-                          (2 * M_PI / X->d.omega);
-        return 1.0 + im * I;
-    } else if (x > (X->d.nx - bx) && X->d.backBC == ORA_PML) {
-        const double delta = x - (X->d.nx - bx);
-        const double im = (X->d.sigmax / etax) * (delta / bx) * (delta / bx) *
-                          (2 * M_PI / X->d.omega);
-        return 1.0 + im * I;
-    }
+    const int back = X->d.nx - bx - (ORA_PLANT == 1);
+    if (x < (bx - 1) && X->d.frontBC == ORA_PML)
+        return pml_value(X->d.sigmax, bx, X->hx, X->d.omega, (bx - 1) - x);
+    else if (x > back && X->d.backBC == ORA_PML)
+        return pml_value(X->d.sigmax, bx, X->hx, X->d.omega, x - back);
     return 1.0;
 }
 
 static zc s2inv(const ora_ctx* X, int y)
 {
     const int by = X->d.by;
-    const double etay = by * X->hy;
-    if (y < (by - 1) && X->d.leftBC == ORA_PML) {
-        const double delta = (by - 1) - y;
-        const double im = (X->d.sigmay / etay) * (delta / by) * (delta / by) *
-                          (2 * M_PI / X->d.omega);
-        return 1.0 + im * I;
-    } else if (y > (X->d.ny - by) && X->d.rightBC == ORA_PML) {
-        const double delta = y - (X->d.ny - by);
-        const double im = (X->d.sigmay / etay) * (delta / by) * (delta / by) *
-                          (2 * M_PI / X->d.omega);
-        return 1.0 + im * I;
-    }
+    if (y < (by - 1) && X->d.leftBC == ORA_PML)
+        return pml_value(X->d.sigmay, by, X->hy, X->d.omega, (by - 1) - y);
+    else if (y > (X->d.ny - by) && X->d.rightBC == ORA_PML)
+        return pml_value(X->d.sigmay, by, X->hy, X->d.omega, y - (X->d.ny - by));
     return 1.0;
 }
 
 static zc s3inv(const ora_ctx* X, int v)
 {
     const int bz = X->d.bz;
-    const double etaz = bz * X->hz;
-    if (v < bz - 1) {
-        const double delta = (bz - 1) - v;
-        const double im = (X->d.sigmaz / etaz) * (delta / bz) * (delta / bz) *
-                          (2 * M_PI / X->d.omega);
-        return 1.0 + im * I;
-    } else if (v > (X->d.nz - bz) && X->d.topBC == ORA_PML) {
-        const double delta = v - (X->d.nz - bz);
-        const double im = (X->d.sigmaz / etaz) * (delta / bz) * (delta / bz) *
-                          (2 * M_PI / X->d.omega);
-        return 1.0 + im * I;
-    }
+    if (v < bz - 1)
+        return pml_value(X->d.sigmaz, bz, X->hz, X->d.omega, (bz - 1) - v);
+    else if (v > (X->d.nz - bz) && X->d.topBC == ORA_PML)
+        return pml_value(X->d.sigmaz, bz, X->hz, X->d.omega, v - (X->d.nz - bz));
     return 1.0;
 }
 
+/* s3InvArtificial (initialize_finalize.hpp:500-525): the panel's own PML
+ * over the bz planes below vOffset + bz - 1. */
 static zc s3inv_art(const ora_ctx* X, int v, int vOffset)
 {
     const int bz = X->d.bz;
-    const double etaz = bz * X->hz;
-    if (v < vOffset + bz - 1) {
-        const double delta = (vOffset + bz - 1) - v;
-        const double im = (X->d.sigmaz / etaz) * (delta / bz) * (delta / bz) *
-                          (2 * M_PI / X->d.omega);
-        return 1.0 + im * I;
-    } else if (v > (X->d.nz - bz) && X->d.topBC == ORA_PML) {
-        const double delta = v - (X->d.nz - bz);
-        const double im = (X->d.sigmaz / etaz) * (delta / bz) * (delta / bz) *
-                          (2 * M_PI / X->d.omega);
-        return 1.0 + im * I;
-    }
+    const int lim = vOffset + bz - 1 + (ORA_PLANT == 4);
+    if (v < lim)
+        return pml_value(X->d.sigmaz, bz, X->hz, X->d.omega, lim - v);
+    else if (v > (X->d.nz - bz) && X->d.topBC == ORA_PML)
+        return pml_value(X->d.sigmaz, bz, X->hz, X->d.omega, v - (X->d.nz - bz));
     return 1.0;
 }
 
@@ -469,7 +464,7 @@ static void global_row(const ora_ctx* X, double alpha, int x, int y, int v,
 static void panel_row(const ora_ctx* X, double alpha, int x, int y, int v,
                       int vOffset, zc out[7])
 {
-    const zc so = X->d.omega + X->d.damping * I;
+    const zc so = X->d.omega + (ORA_PLANT == 5 ? -X->d.damping : X->d.damping) * I;
     const zc sa = so / alpha;
     stencil(X, s1inv(X, x - 1), s1inv(X, x), s1inv(X, x + 1), s2inv(X, y - 1),
             s2inv(X, y), s2inv(X, y + 1), s3inv_art(X, v - 1, vOffset),

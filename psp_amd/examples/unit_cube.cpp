@@ -3,12 +3,18 @@
 // right-hand sides (point source, three point sources, Gaussian beam, plane
 // wave; UnitCube.cpp:485-542) with GMRES(20) to 1e-5, report.
 //
-//   unit_cube [n=40] [omega=30] [model=2] [fp32=0]
+//   unit_cube [n=40] [omega=30] [model=2] [fp32=0] [prefix]
 // models: 0 uniform, 1 converging lens, 2 wave guide (UnitCube.cpp:91-157).
+// With a prefix, the velocity, the right-hand sides and the solution are
+// written as raw native-endian doubles (grid layout, complex interleaved) to
+// <prefix>_velocity.bin, <prefix>_rhs.bin and <prefix>_solution.bin, and the
+// object is Finalized and Initialized once more before the solve (the
+// reference's Finalize keeps the object usable).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "psp.hpp"
 
@@ -22,6 +28,16 @@ int main(int argc, char* argv[])
     const double omega = argc > 2 ? std::atof(argv[2]) : 30.;
     const int model = argc > 3 ? std::atoi(argv[3]) : 2;
     const bool fp32 = argc > 4 && std::atoi(argv[4]) != 0;
+    const char* prefix = argc > 5 ? argv[5] : nullptr;
+    auto dump = [&](const char* what, const void* data, size_t bytes) {
+        const std::string name = std::string(prefix) + "_" + what + ".bin";
+        FILE* f = std::fopen(name.c_str(), "wb");
+        if (!f || std::fwrite(data, 1, bytes, f) != bytes) {
+            std::fprintf(stderr, "could not write %s\n", name.c_str());
+            std::exit(3);
+        }
+        std::fclose(f);
+    };
     const int pmlSize = 5;
     const double sigma = 1.5, damping = 7.;
     const int planesPerPanel = 4;
@@ -45,6 +61,10 @@ int main(int argc, char* argv[])
                 }
         auto t0 = std::chrono::steady_clock::now();
         helmholtz.Initialize(velocity);
+        if (prefix) {
+            helmholtz.Finalize();
+            helmholtz.Initialize(velocity);
+        }
         const double tInit =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
@@ -73,11 +93,17 @@ int main(int argc, char* argv[])
                                        y < n - pmlSize && z >= pmlSize && z < n - pmlSize;
                     p[3] = inner ? -plane : C(0);
                 }
+        const size_t pts = (size_t)n * n * n;
+        if (prefix) {
+            dump("velocity", v, sizeof(double) * pts);
+            dump("rhs", b, sizeof(C) * 4 * pts);
+        }
         t0 = std::chrono::steady_clock::now();
         helmholtz.Solve(B, 20, 1e-5);
         const double tSolve =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const psp_hip_solve_report& r = helmholtz.SolveReport();
+        if (prefix) dump("solution", b, sizeof(C) * 4 * pts);
         std::printf("unit_cube n=%d omega=%g model=%d fronts=%s: initialize %.3f s, solve %.3f s, "
                     "%d GMRES iterations, max rel residual %.3e\n",
                     n, omega, model, fp32 ? "fp32" : "fp64", tInit, tSolve, r.iterations,
