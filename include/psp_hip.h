@@ -179,6 +179,16 @@ int psp_hip_multiply(psp_hip_ctx* ctx, const void* x, void* y, int k);
  * block in grid layout. */
 int psp_hip_precondition(psp_hip_ctx* ctx, void* B, int k);
 
+/* Precondition without allocating (no reference counterpart: for callers
+ * that cannot allocate device memory, such as code under HIP graph capture
+ * or serving loops that forbid it).  Same result as psp_hip_precondition,
+ * with the sweep's buffers in a fixed 64 MiB device array of the library
+ * (one per device, held until the process ends) instead of the context's
+ * workspace; calls no hipMalloc or hipFree and replays no graph.  A grid and
+ * k whose sweep needs more than 64 MiB is PSP_HIP_ERR_INVALID.  Calls that
+ * overlap on one device share the array and race: serialize them. */
+int psp_hip_precondition_noalloc(psp_hip_ctx* ctx, void* B, int k);
+
 /* DistHelmholtz::SolvePanel (solve.hpp:736-882) in place on a device block
  * in grid layout (only panel `panel`'s planes are read and written). */
 int psp_hip_solve_panel(psp_hip_ctx* ctx, void* B, int k, int panel);

@@ -29,6 +29,7 @@ EXPORTS = (
     "psp_hip_last_error", "psp_hip_set_stream",
     "psp_hip_set_option",
     "psp_hip_factor", "psp_hip_solve", "psp_hip_multiply", "psp_hip_precondition",
+    "psp_hip_precondition_noalloc",
     "psp_hip_solve_panel", "psp_hip_solve_fixed", "psp_hip_adjoint_gradient",
     "psp_hip_profile_sweep", "psp_hip_solve_batch", "psp_hip_autotune",
     "psp_hip_num_panels", "psp_hip_panel_info", "psp_hip_class_num_nodes",
@@ -116,6 +117,7 @@ def lib():
         "psp_hip_autotune": (I, [P, I, C.POINTER(D), C.POINTER(D)]),
         "psp_hip_multiply": (I, [P, P, P, I]),
         "psp_hip_precondition": (I, [P, P, I]),
+        "psp_hip_precondition_noalloc": (I, [P, P, I]),
         "psp_hip_solve_panel": (I, [P, P, I, I]),
         "psp_hip_solve_fixed": (I, [P, P, I, I, I, I, P]),
         "psp_hip_adjoint_gradient": (I, [P, P, P, I, P, I]),

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <algorithm>
 #include <vector>
@@ -72,6 +73,7 @@ struct psp_hip_ctx {
     std::vector<SweepGraph> graphs;
     psp_hip_iterate_fn iterCb = nullptr;  // called with every unconverged iterate
     void* iterUser = nullptr;
+    void* scratch = nullptr;  // device address of this device's no-allocation scratch
 };
 
 namespace {
@@ -215,6 +217,38 @@ std::vector<SolveTile> SolveTiles(const ClassPlan& P, const std::vector<MvTile>&
     return out;
 }
 
+// Scratch of psp_hip_precondition_noalloc, the sweep entry point that
+// allocates nothing: the sweep buffers (SweepBufferSizes) live in this
+// file-scope device array, one copy per device, held until the process
+// ends.  Host code never uses the array's name as an address: each device's
+// copy is looked up once (ScratchAddress, at the context's device set-up)
+// and that address is what kernels and fills receive.  Calls that overlap on
+// one device share the array and race.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+void* gScratchAddr[kMaxDevices] = {};
+
+// Device address of the current device's scratch copy (looked up once per
+// device, with that device current).
+int ScratchAddress(psp_hip_ctx* c, void** out)
+{
+    *out = nullptr;
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return Fail(c, PSP_HIP_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchAddr[dev]) {
+        void* p = nullptr;
+        HIPCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        gScratchAddr[dev] = p;
+    }
+    *out = gScratchAddr[dev];
+    return PSP_HIP_OK;
+}
+
 int EnsureDevice(psp_hip_ctx* c)
 {
     if (c->devReady) return PSP_HIP_OK;
@@ -262,6 +296,7 @@ int EnsureDevice(psp_hip_ctx* c)
     int dev = 0, cus = 0;
     HIPCHK(c, hipGetDevice(&dev));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if ((st = ScratchAddress(c, &c->scratch))) return st;
     PrimeSolveOccupancy();
     for (int cl = 0; cl < 3; ++cl) {
         if (!c->plan.present[cl]) continue;
@@ -400,11 +435,17 @@ int64_t NeededSlots(const psp_hip_ctx* c)
     return maxs;
 }
 
-int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
+// Byte sizes of the sweep's buffers for k right-hand sides: the padded
+// panel vector Xp, the node work vectors W, the partial sums P of split
+// solve row blocks and, per panel class, the two slot copies G of the
+// children's update rows.  Vectors and slot copies end in kSolvePad + 2
+// rows that must read as zero (the solve kernels read past a tile's rows).
+struct SweepSizes {
+    size_t xp = 0, w = 0, p = 0, g[3] = {0, 0, 0};
+};
+
+SweepSizes SweepBufferSizes(const psp_hip_ctx* c, int k)
 {
-    if (c->wsK == k && c->wsM >= m && c->wsSlots >= NeededSlots(c)) return PSP_HIP_OK;
-    FreeWorkspace(c);
-    const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
     const size_t eb = FrontBytes(c);
     int64_t maxn = 0, maxw = 0, maxs = 1;
     for (int cl = 0; cl < 3; ++cl)
@@ -413,23 +454,37 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
             maxw = std::max<int64_t>(maxw, c->plan.cls[cl].wstore);
             maxs = std::max<int64_t>(maxs, c->plan.cls[cl].maxSlots);
         }
+    const size_t padRows = (size_t)kSolvePad + 2;
+    SweepSizes z;
+    z.xp = eb * (maxn + padRows) * k;
+    z.w = eb * (maxw + padRows) * k;
+    z.p = eb * maxs * 64 * k;
+    for (int cl = 0; cl < 3; ++cl)
+        if (c->plan.present[cl])
+            z.g[cl] = eb * (2 * (size_t)c->plan.cls[cl].wstore + padRows) * k;
+    return z;
+}
+
+int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
+{
+    if (c->wsK == k && c->wsM >= m && c->wsSlots >= NeededSlots(c)) return PSP_HIP_OK;
+    FreeWorkspace(c);
+    const int64_t N = (int64_t)c->desc.nx * c->desc.ny * c->desc.nz;
+    const SweepSizes z = SweepBufferSizes(c, k);
     int st;
     void* p;
-    // vectors end in kSolvePad + 2 zeroed rows (reads past a tile's rows)
-    const size_t padRows = (size_t)kSolvePad + 2;
-    if ((st = Alloc(c, &c->Xp, eb * (maxn + padRows) * k, c->wsAllocs))) return st;
-    if ((st = Alloc(c, &c->W, eb * (maxw + padRows) * k, c->wsAllocs))) return st;
-    HIPCHK(c, hipMemsetAsync(c->Xp, 0, eb * (maxn + padRows) * k, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->W, 0, eb * (maxw + padRows) * k, c->stream));
-    if ((st = Alloc(c, &c->P, eb * maxs * 64 * k, c->wsAllocs))) return st;
+    if ((st = Alloc(c, &c->Xp, z.xp, c->wsAllocs))) return st;
+    if ((st = Alloc(c, &c->W, z.w, c->wsAllocs))) return st;
+    HIPCHK(c, hipMemsetAsync(c->Xp, 0, z.xp, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->W, 0, z.w, c->stream));
+    if ((st = Alloc(c, &c->P, z.p, c->wsAllocs))) return st;
     for (int cl = 0; cl < 3; ++cl) {
         c->G[cl] = nullptr;
         if (!c->plan.present[cl]) continue;
         // Zero once: each class writes the same rows of its slot copies in
         // every solve, and the rows it never writes must read as zero.
-        const size_t gb = eb * (2 * (size_t)c->plan.cls[cl].wstore + padRows) * k;
-        if ((st = Alloc(c, &c->G[cl], gb, c->wsAllocs))) return st;
-        HIPCHK(c, hipMemsetAsync(c->G[cl], 0, gb, c->stream));
+        if ((st = Alloc(c, &c->G[cl], z.g[cl], c->wsAllocs))) return st;
+        HIPCHK(c, hipMemsetAsync(c->G[cl], 0, z.g[cl], c->stream));
     }
     if ((st = Alloc(c, &p, sizeof(c128) * N * k * m, c->wsAllocs))) return st;
     c->V = (c128*)p;
@@ -454,7 +509,7 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
     c->ydev = (c128*)p;
     c->wsK = k;
     c->wsM = m;
-    c->wsSlots = maxs;
+    c->wsSlots = NeededSlots(c);
     return PSP_HIP_OK;
 }
 
@@ -1237,6 +1292,44 @@ int psp_hip_precondition(psp_hip_ctx* c, void* B, int k)
     if (rc) return rc;
     if ((rc = EnsureWorkspace(c, k, std::max(1, c->wsK == k ? c->wsM : 1)))) return rc;
     if ((rc = Precondition(c, (c128*)B, k))) return rc;
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return PSP_HIP_OK;
+}
+
+int psp_hip_precondition_noalloc(psp_hip_ctx* c, void* B, int k)
+{
+    int rc = CheckSolveArgs(c, B, k, 1);
+    if (rc) return rc;
+    if (!c->scratch) return Fail(c, PSP_HIP_ERR_STATE, "device scratch not set up");
+    const SweepSizes z = SweepBufferSizes(c, k);
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t need = up(z.xp) + up(z.w) + up(z.p) + up(z.g[0]) + up(z.g[1]) + up(z.g[2]);
+    if (need > kScratchBytes)
+        return Fail(c, PSP_HIP_ERR_INVALID,
+                    "precondition_noalloc: the sweep needs " + std::to_string(need) +
+                        " bytes of scratch for this grid and k; the maximum is " +
+                        std::to_string(kScratchBytes) + " (use psp_hip_precondition)");
+    char* base = (char*)c->scratch;
+    // the slot copies and the vectors' padding rows must read as zero; a
+    // fill kernel clears the whole region (no memset node if captured)
+    LaunchFillZero(base, (int64_t)need, c->stream);
+    void* keep[6] = {c->Xp, c->W, c->P, c->G[0], c->G[1], c->G[2]};
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        void* q = b ? base + off : nullptr;
+        off += up(b);
+        return q;
+    };
+    c->Xp = take(z.xp);
+    c->W = take(z.w);
+    c->P = take(z.p);
+    for (int cl = 0; cl < 3; ++cl) c->G[cl] = take(z.g[cl]);
+    SweepEager(c, (c128*)B, k);
+    c->Xp = keep[0];
+    c->W = keep[1];
+    c->P = keep[2];
+    for (int cl = 0; cl < 3; ++cl) c->G[cl] = keep[3 + cl];
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PSP_HIP_OK;

@@ -432,6 +432,16 @@ class DistHelmholtz:
         if st != _lib.OK:
             _raise(st, self._h, "Precondition")
 
+    def PreconditionNoAlloc(self, B):
+        """Precondition with the sweep's buffers in the library's fixed
+        device scratch (psp_hip_precondition_noalloc): allocates nothing;
+        grids whose sweep needs more than the scratch are a LogicError."""
+        self._bind_stream()
+        st = _lib.lib().psp_hip_precondition_noalloc(self._ctx(), C.c_void_p(B.data_ptr()),
+                                                     B.shape[1])
+        if st != _lib.OK:
+            _raise(st, self._h, "PreconditionNoAlloc")
+
     def SolvePanel(self, B, panel):
         self._bind_stream()
         st = _lib.lib().psp_hip_solve_panel(self._ctx(), C.c_void_p(B.data_ptr()), B.shape[1],
