@@ -64,10 +64,25 @@ enum {
                                         solves with persistent, pipelined
                                         workgroups (8 <= k <= 16); 0: one
                                         workgroup per tile */
-    PSP_HIP_OPT_SOLVE_PERSIST_GRID = 6 /* persistent launches use at most this many
+    PSP_HIP_OPT_SOLVE_PERSIST_GRID = 6, /* persistent launches use at most this many
                                           workgroups (more only when one would
                                           get over 32 tiles); 0 (default): as many
                                           as fit on the device at once */
+    PSP_HIP_OPT_SOLVE_FUSED = 7,       /* 1: for 8 <= k <= 16, the tree levels from
+                                          the cut depth down run as one persistent
+                                          launch per direction, a subtree queue per
+                                          XCD; 0 (default): one launch per level */
+    PSP_HIP_OPT_SOLVE_FUSED_QUEUES = 8, /* queues of the fused launches (default 8,
+                                          one per XCD; more leaves queues that the
+                                          last workgroup runs alone) */
+    PSP_HIP_OPT_SOLVE_FUSED_ROWS = 9,  /* output rows per fused tile: 16, 32 (default)
+                                          or 64 */
+    PSP_HIP_OPT_SOLVE_FUSED_CUT = 10,  /* cut depth of the fused launches; -1 (default):
+                                          the first depth with 8 nodes */
+    PSP_HIP_OPT_SOLVE_FUSED_WG = 11    /* workgroups a fused queue expects (default 64):
+                                          near the cut, row blocks are cut into inner
+                                          slices until a level has about that many
+                                          tiles per queue; 1: no slices */
 };
 
 /* Flags for psp_hip_solve. */

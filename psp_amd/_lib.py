@@ -22,6 +22,11 @@ OPT_SOLVE_MAX_SPLIT = 3
 OPT_SOLVE_WIDE_WG = 4
 OPT_SOLVE_PERSIST = 5
 OPT_SOLVE_PERSIST_GRID = 6
+OPT_SOLVE_FUSED = 7
+OPT_SOLVE_FUSED_QUEUES = 8
+OPT_SOLVE_FUSED_ROWS = 9
+OPT_SOLVE_FUSED_CUT = 10
+OPT_SOLVE_FUSED_WG = 11
 
 # Every entry point include/psp_hip.h declares.
 EXPORTS = (

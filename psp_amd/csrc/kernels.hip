@@ -1197,6 +1197,382 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
 }
 
 // --------------------------------------------------------------------------
+// Fused subtree solves (8 <= k <= KB).  One persistent launch runs every
+// level of a phase from the class's cut depth down (forward: deepest level
+// first; backward: the cut level first).  The subtrees under the cut nodes
+// are split over nqueues queues, one per XCD: a workgroup reads the XCD it
+// runs on (hardware register XCC_ID), serves queue XCC_ID (if there is
+// one; no two XCDs ever serve one queue), takes its tiles in
+// queue order (an atomic head), and before reading a tile's inputs waits
+// until the nodes the tile depends on have all their row blocks done
+// (per-node counters; forward: the node's children, backward: its nearest
+// ancestor with pivots).  Producer and consumer of every hand-off then
+// share one XCD and its L2, and the hand-off is the cross-CU one of the
+// memory model, with no L2 write-back:
+//   producer: its stores; s_waitcnt vmcnt(0) (the stores have reached the
+//             L2); barrier; one lane increments the node's counter;
+//   consumer: polls the counter; barrier; reads what the launch wrote with
+//             loads that miss the L1 and read the L2 (sc1, L2Buf).
+// Counters and heads are device-scope atomics.  Near the cut a row block
+// is cut into inner slices: slices write partial sums to their slots, and
+// the slice that arrives last (a counter per row block) sums the slots in
+// slot order and finishes the rows, so sums have a fixed order.
+//
+// No workgroup waits forever: a tile's dependencies come earlier in its
+// queue, and the earliest tile not yet done is held by a running workgroup
+// whose own earlier tiles are done, so it proceeds.  A queue that no
+// workgroup serves (fewer XCDs than queues) is run, tile by tile, by the
+// last workgroup to leave (an exit counter).  The per-tile body is that of
+// k_mvm: waves own 16-row blocks and every WK-th 4-column step, the inner
+// range streams through LDS, factor loads run U steps ahead; the first
+// factor batch of a tile is issued before its dependency wait.  The launch
+// state (counters, heads) is cleared by the copy-in or zero-set launch
+// that precedes each panel solve.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int XccId()
+{
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);  // HW_REG_XCC_ID, bits 3:0
+}
+
+__device__ __forceinline__ int LoadAgent(const int32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int AddAgent(int32_t* p, int v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// This wave's stores (and loads) have completed: s_waitcnt vmcnt(0).
+__device__ __forceinline__ void WaitVmem() { __builtin_amdgcn_s_waitcnt(0xF70); }
+
+// Loads of vectors that another workgroup of the XCD wrote during this
+// launch: buffer loads with the device-scope cache policy (sc1), which miss
+// the CU's L1 and read the L2, where the producer's stores are.  They are
+// ordinary loads to the compiler (scheduled and batched like any other).
+// Measured on MI355X: with sc0 loads (workgroup scope), or with an L1
+// invalidate after the wait, results went wrong whenever a vector row was
+// not a whole number of 128-byte lines (a line then holds rows of two nodes,
+// and the L1 can keep or regain a copy read before the other node was
+// written); with sc1 loads every case is exact, and so are full
+// device-scope release/acquire fences, which also write back and
+// invalidate the L2.  A buffer spans at most 4 GiB (the host checks).
+template <typename T>
+struct L2Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ explicit L2Buf(const void* base)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000))
+    {
+    }
+    __device__ cx<T> operator[](int64_t i) const
+    {
+        constexpr int kSc1 = 16;  // cache policy: SC1 set (device scope)
+        const uint32_t off = (uint32_t)(i * (int64_t)sizeof(cx<T>));
+        cx<T> v;
+        if constexpr (sizeof(T) == 4) {
+            const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kSc1);
+            __builtin_memcpy(&v, &x, 8);
+        } else {
+            const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kSc1);
+            __builtin_memcpy(&v, &x, 16);
+        }
+        return v;
+    }
+};
+
+// The fused launch's buffers read through the L2: slot copies, work vectors,
+// panel vector, partial sums.
+template <typename T>
+struct L2Bufs {
+    L2Buf<T> G, W, X, P;
+};
+
+// StageBatch of the fused launch: the same operand (the same sums in the
+// same order) with the vectors written in this launch read from the L2.
+// Backward: both the work-vector and the panel-vector element are loaded
+// (a buffer per vector, so no per-lane choice of descriptor) and one kept.
+template <typename T, int PH, int N, class AT>
+__device__ __forceinline__ void StageBatchL2(const NodeDev& nd, const int32_t* structs,
+                                             const Slots<T>& gs, const L2Bufs<T>& lb,
+                                             const cx<T>* Xp, int k, const AT& at,
+                                             cx<T> (&sv)[N])
+{
+    int32_t sx[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) sx[q] = StageIndex<PH>(nd, structs, at(q).x);
+    if (PH == PH_B) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) sx[q] = Opaque(sx[q]);
+    }
+    // all loads first, combined once they have arrived
+    cx<T> ra[N], rb[N], rc[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const int2 p = at(q);
+        const int col = p.x, rr = p.y;
+        if (PH == PH_F) {
+            const int64_t g = (nd.w + col) * k + rr;
+            ra[q] = (lb.G[g]);
+            rb[q] = (lb.G[gs.half + g]);
+            rc[q] = (Xp[(int64_t)(nd.off + col) * k + rr]);
+        } else {
+            ra[q] = (lb.W[(nd.w + min(col, nd.s - 1)) * k + rr]);
+            rb[q] = (lb.X[(int64_t)sx[q] * k + rr]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        if (PH == PH_F)
+            sv[q] = ra[q] + rb[q] + rc[q];
+        else
+            sv[q] = at(q).x < nd.s ? ra[q] : rb[q];
+    }
+}
+
+template <typename T, int PH, int RT, int KB>
+__global__ __launch_bounds__(256) void k_mvq(const QTile* tiles, const int32_t* qstart,
+                                             const QRed* reds, const int4* dep,
+                                             const NodeDev* nodes, int32_t* sync, int nnodes,
+                                             int nqueues, const int32_t* structs, Slots<T> gs,
+                                             const cx<T>* L, const cx<T>* LT, cx<T>* Xp,
+                                             cx<T>* W, cx<T>* P, int k)
+{
+    typedef typename Mfma<T>::acc acc_t;
+    constexpr int NWV = 4, NT = 256, WR = RT / 16, WK = NWV / WR, NB = KB / 8;
+    constexpr int U = sizeof(T) == 4 ? PSP_MVM_U32 : PSP_MVM_U64;
+    constexpr int CC = (sizeof(T) == 4 ? PSP_MVM_CHUNK32 : PSP_MVM_CHUNK64) / KB;
+    constexpr int SPT = CC * KB / NT;
+    constexpr bool PRE = PH == PH_F;
+    static_assert(WR * WK == NWV && CC % 4 == 0 && SPT >= 1, "tile shape");
+    static_assert(WK == 1 || 2 * CC * KB >= WR * (WK / 2) * NB * 256,
+                  "reduction scratch must fit in vs");
+    int32_t* done = sync;
+    int32_t* head = sync + nnodes;
+    int32_t* exitc = head + nqueues;
+    int32_t* redc = exitc + 1;
+    const L2Bufs<T> lb{L2Buf<T>(gs.G), L2Buf<T>(W), L2Buf<T>(Xp), L2Buf<T>(P)};
+    __shared__ cx<T> vs[CC][KB];
+    __shared__ int bc[4];
+    T* red = reinterpret_cast<T*>(&vs[0][0]);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave / WK, wk = wave % WK;
+    const int li = lane & 15, lk = lane >> 4;
+
+    // One queue tile: wait for its dependencies, the products, the finish
+    // (or the partial sums and, for the last slice, the row block's sum and
+    // finish), then the node's counter.
+    auto runTile = [&](int idx) {
+        const QTile qt = tiles[idx];
+        const NodeDev nd = nodes[qt.node];
+        const int s = nd.s, fs = nd.s + nd.u;
+        const int fa = PH == PH_F ? fs : s;
+        const cx<T>* Ab0 = (PH == PH_F ? L : LT) + nd.l + qt.row0;
+        const uint32_t la = (uint32_t)(min(16 * wr + li, qt.nrows - 1) + (4 * wk + lk) * fa);
+        const int64_t ustep = (int64_t)(4 * WK) * fa;
+        // the first factor batch (independent of the dependencies)
+        const int ncol0 = min(CC, qt.col1 - qt.col0);
+        const int nsteps0 = (max(ncol0, 0) + 3) >> 2;
+        const int mine0 = nsteps0 > wk ? (nsteps0 - wk + WK - 1) / WK : 0;
+        cx<T> m[U];
+        {
+            const cx<T>* Ab = Ab0 + (int64_t)qt.col0 * fa;
+            const int ulast = max(mine0 - 1, 0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) m[u] = (Ab + min(u, ulast) * ustep)[la];
+        }
+        int prow[4] = {0, 0, 0, 0};
+        const bool fin = wk == 0 && li < 8 && qt.red < 0;
+        if (PRE && fin) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = min(qt.row0 + 16 * wr + Mfma<T>::row(lk, q), fs - 1);
+                prow[q] = gs.relinds[nd.ustart + max(row - s, 0)];
+            }
+        }
+        // dependencies
+        {
+            const int4 dp = dep[qt.node];
+            if (tid == 0) {
+                if (dp.x >= 0)
+                    while (LoadAgent(done + dp.x) < dp.z) __builtin_amdgcn_s_sleep(1);
+                if (dp.y >= 0)
+                    while (LoadAgent(done + dp.y) < dp.w) __builtin_amdgcn_s_sleep(1);
+            }
+            LdsBarrier();
+        }
+        cx<T> pre[PRE ? NB : 1][4], pre1[PRE ? NB : 1][4];
+        if (PRE && fin) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = min(qt.row0 + 16 * wr + Mfma<T>::row(lk, q), fs - 1);
+                    const int64_t o = (nd.w + row) * k + min(8 * b + li, k - 1);
+                    pre[PRE ? b : 0][q] = (lb.G[o]);
+                    pre1[PRE ? b : 0][q] = (lb.G[gs.half + o]);
+                }
+        }
+        acc_t acc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = acc_t{0, 0, 0, 0};
+        for (int cb = qt.col0; cb < qt.col1; cb += CC) {
+            const int ncol = min(CC, qt.col1 - cb);
+            const int nsteps = (ncol + 3) >> 2;
+            const int mine = nsteps > wk ? (nsteps - wk + WK - 1) / WK : 0;
+            const cx<T>* Ab = Ab0 + (int64_t)cb * fa;
+            const int ulast = max(mine - 1, 0);
+            auto load = [&](cx<T>* dst, int u0) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) dst[u] = (Ab + min(u0 + u, ulast) * ustep)[la];
+            };
+            const int nst = min(CC, 4 * nsteps);
+            cx<T> sv[SPT];
+            StageBatchL2<T, PH>(nd, structs, gs, lb, Xp, k, [&](int q) {
+                const int e = tid + NT * q, c = e / KB, r = e % KB;
+                return int2{cb + min(c, ncol - 1), min(r, k - 1)};
+            }, sv);
+            if (cb != qt.col0) load(m, 0);
+            LdsBarrier();  // the previous chunk's readers are done with vs
+#pragma unroll
+            for (int q = 0; q < SPT; ++q) {
+                const int e = tid + NT * q, c = e / KB, r = e % KB;
+                if (c < nst) vs[c][r] = (r < k && c < ncol) ? sv[q] : czero<T>();
+            }
+            LdsBarrier();
+            for (int u0 = 0; u0 < mine; u0 += U) {
+                cx<T> mn[U];
+                if (u0 + U < mine) load(mn, u0 + U);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (u0 + u < mine) {
+                        const int c = 4 * (wk + WK * (u0 + u)) + lk;
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) {
+                            const cx<T> v = vs[c][8 * b + (li & 7)];
+                            const T b1 = li < 8 ? v.r : v.i;
+                            const T b2 = li < 8 ? -v.i : v.r;
+                            acc[b] = Mfma<T>::step(m[u].r, b1, acc[b]);
+                            acc[b] = Mfma<T>::step(m[u].i, b2, acc[b]);
+                        }
+                    }
+                }
+                if (u0 + U < mine) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) m[u] = mn[u];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) Sink(m[u].r), Sink(m[u].i);
+        if (WK > 1) {  // waves that share rows sum their partial tiles by a fixed tree
+            for (int h = WK / 2; h >= 1; h >>= 1) {
+                LdsBarrier();
+                if (wk >= h && wk < 2 * h) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            red[(((wr * h + wk - h) * NB + b) * 64 + lane) * 4 + q] = acc[b][q];
+                }
+                LdsBarrier();
+                if (wk < h) {
+#pragma unroll
+                    for (int b = 0; b < NB; ++b)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            acc[b][q] += red[(((wr * h + wk) * NB + b) * 64 + lane) * 4 + q];
+                }
+            }
+        }
+        if (wk == 0) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const T val = acc[b][q];
+                    const T other = __shfl_xor(val, 8, 64);
+                    const int rl = 16 * wr + Mfma<T>::row(lk, q);
+                    const int rr = 8 * b + li;
+                    if (li < 8 && rl < qt.nrows && rr < k) {
+                        const cx<T> a{val, other};
+                        if (qt.red >= 0)
+                            P[((int64_t)qt.slot * 64 + rl) * k + rr] = a;
+                        else if (PRE)
+                            Finish<T, PH>(nd, gs, Xp, W, qt.row0 + rl, rr, k, a,
+                                          pre[PRE ? b : 0][q] + pre1[PRE ? b : 0][q], prow[q]);
+                        else
+                            Finish<T, PH>(nd, gs, Xp, W, qt.row0 + rl, rr, k, a, czero<T>(), 0);
+                    }
+                }
+            }
+        }
+        WaitVmem();
+        LdsBarrier();
+        bool rowsDone = qt.red < 0;
+        if (!rowsDone) {
+            const QRed rd = reds[qt.red];
+            if (tid == 0) bc[2] = AddAgent(redc + qt.red, 1);
+            LdsBarrier();
+            if (__builtin_amdgcn_readfirstlane(bc[2]) == rd.nslots - 1) {
+                // the last slice: every other slice's partials are in L2
+                for (int e = tid; e < rd.nrows * k; e += NT) {
+                    const int rl = e / k, r = e % k;
+                    cx<T> a = czero<T>();
+                    for (int j = 0; j < rd.nslots; ++j)
+                        a = a + lb.P[((int64_t)(rd.slot0 + j) * 64 + rl) * k + r];
+                    if (PH == PH_F && rd.row0 + rl >= nd.s) {
+                        const int row = rd.row0 + rl;
+                        const int64_t g = (nd.w + row) * k + r;
+                        Finish<T, PH>(nd, gs, Xp, W, row, r, k, a, lb.G[g] + lb.G[gs.half + g],
+                                      gs.relinds[nd.ustart + row - nd.s]);
+                    } else {
+                        Finish<T, PH>(nd, gs, Xp, W, rd.row0 + rl, r, k, a, czero<T>(), 0);
+                    }
+                }
+                WaitVmem();
+                LdsBarrier();
+                rowsDone = true;
+            }
+        }
+        if (rowsDone && tid == 0) AddAgent(done + qt.node, 1);
+    };
+
+    // queue x is served by the workgroups of XCD x only (none of another XCD
+    // ever shares it); XCDs without a queue go straight to the exit
+    const int q = __builtin_amdgcn_readfirstlane(XccId());
+    int cur = 0, qs = 0, qn = 0;
+    if (q < nqueues) {
+        qs = qstart[q];
+        qn = qstart[q + 1] - qs;
+        if (tid == 0) bc[0] = AddAgent(head + q, 1);
+        LdsBarrier();
+        cur = __builtin_amdgcn_readfirstlane(bc[0]);
+    }
+    while (cur < qn) {
+        int nxt = 0;
+        if (tid == 0) nxt = AddAgent(head + q, 1);  // the tile after this one
+        runTile(qs + cur);
+        if (tid == 0) bc[1] = nxt;
+        LdsBarrier();
+        cur = __builtin_amdgcn_readfirstlane(bc[1]);
+    }
+    // the last workgroup out runs the queues no workgroup served
+    if (tid == 0) bc[3] = AddAgent(exitc, 1);
+    LdsBarrier();
+    if (__builtin_amdgcn_readfirstlane(bc[3]) != (int)gridDim.x - 1) return;
+    for (int qq = 0; qq < nqueues; ++qq) {
+        if (tid == 0) bc[0] = LoadAgent(head + qq);
+        LdsBarrier();
+        const int served = __builtin_amdgcn_readfirstlane(bc[0]);
+        LdsBarrier();
+        if (served != 0) continue;
+        for (int i = qstart[qq]; i < qstart[qq + 1]; ++i) runTile(i);
+    }
+}
+
+
+// --------------------------------------------------------------------------
 // Complex tile engine on the matrix cores: one 256-thread workgroup computes
 // an RT x 64 complex tile C = A B over an inner range, K staged through LDS
 // 16 columns at a time with two buffers (the global loads of stage s + 1
@@ -1630,9 +2006,12 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_copy_in(int n, int k, const int32_t* qxy,
                                                        const int32_t* qv, const c128* B,
                                                        cx<T>* Xp, int nx, int ny, int nz,
-                                                       int vOffset, int padding)
+                                                       int vOffset, int padding,
+                                                       int32_t* sync, int nsync)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the fused-launch state of the panel solve that follows starts at zero
+    for (int64_t j = e; j < nsync; j += (int64_t)gridDim.x * blockDim.x) sync[j] = 0;
     if (e >= (int64_t)n * k) return;
     const int q = (int)(e / k), r = (int)(e - (int64_t)q * k);
     const int vl = qv[q];
@@ -1689,9 +2068,11 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void k_zero_set_plane(int n, int k, const int32_t* qxy,
                                                               const int32_t* qv, StencilArgs st,
                                                               const double* vel, const c128* B,
-                                                              cx<T>* Xp, int v2, int vlLast)
+                                                              cx<T>* Xp, int v2, int vlLast,
+                                                              int32_t* sync, int nsync)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t j = e; j < nsync; j += (int64_t)gridDim.x * blockDim.x) sync[j] = 0;
     if (e >= (int64_t)n * k) return;
     const int q = (int)(e / k), r = (int)(e - (int64_t)q * k);
     cx<T> out = czero<T>();
@@ -2029,6 +2410,21 @@ int MvpGrid(int cus, int count, int cap0 = 0)
     return std::max(1, std::min(count, cap));
 }
 
+// Workgroups of a fused launch: as many as fit on the device at once.
+template <typename T, int PH, int RT, int KB>
+int MvqGrid(int cus)
+{
+    static const int perCU = [] {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_mvq<T, PH, RT, KB>, 256, 0) !=
+                hipSuccess ||
+            b < 1)
+            b = 1;
+        return b;
+    }();
+    return perCU * std::max(cus, 1);
+}
+
 void PrimeSolveOccupancy()
 {
     auto prime = [](auto t) {
@@ -2039,6 +2435,12 @@ void PrimeSolveOccupancy()
         (void)MvpGrid<T, PH_B, 64, 8>(1, 1), (void)MvpGrid<T, PH_B, 32, 8>(1, 1);
         (void)MvpGrid<T, PH_B, 16, 8>(1, 1), (void)MvpGrid<T, PH_B, 64, 16>(1, 1);
         (void)MvpGrid<T, PH_B, 32, 16>(1, 1), (void)MvpGrid<T, PH_B, 16, 16>(1, 1);
+        (void)MvqGrid<T, PH_F, 64, 8>(1), (void)MvqGrid<T, PH_F, 32, 8>(1);
+        (void)MvqGrid<T, PH_F, 16, 8>(1), (void)MvqGrid<T, PH_F, 64, 16>(1);
+        (void)MvqGrid<T, PH_F, 32, 16>(1), (void)MvqGrid<T, PH_F, 16, 16>(1);
+        (void)MvqGrid<T, PH_B, 64, 8>(1), (void)MvqGrid<T, PH_B, 32, 8>(1);
+        (void)MvqGrid<T, PH_B, 16, 8>(1), (void)MvqGrid<T, PH_B, 64, 16>(1);
+        (void)MvqGrid<T, PH_B, 32, 16>(1), (void)MvqGrid<T, PH_B, 16, 16>(1);
     };
     prime(float());
     prime(double());
@@ -2047,7 +2449,7 @@ void PrimeSolveOccupancy()
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv, void* Wv, void* Pv,
                       void* Gv, int k, hipStream_t s, MvProfile* prof, int onlyLevel,
-                      int onlyPhase)
+                      int onlyPhase, int32_t* sync)
 {
     const cx<T>* L = (const cx<T>*)c.L + (int64_t)b * c.fstore;
     const cx<T>* LT = (const cx<T>*)c.LT + (int64_t)b * c.fstore;
@@ -2142,14 +2544,64 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
             mv(std::integral_constant<int, PH_B>(), lv, lv.b);
         return;
     }
-    for (int d = plan.maxDepth; d >= 0; --d) {
+    // Fused subtree launches: the levels from the cut down as one launch
+    // per phase (8 <= k <= 16, with the class's launch state to count on).
+    // (the fused kernel reads its vectors through buffer descriptors: each
+    // buffer must span less than 4 GiB)
+    const int64_t eb_ = sizeof(cx<T>), span = (int64_t)1 << 32;
+    const bool fits = eb_ * k * (2 * c.wstore + kSolvePad + 2) < span &&
+                      eb_ * k * (int64_t)(c.n + kSolvePad + 2) < span &&
+                      eb_ * k * 64 * (int64_t)plan.maxSlots < span;
+    const bool fused = sync && c.nqueues > 0 && plan.qcut >= 0 && k >= 8 && k <= 16 && fits;
+    auto fusedLaunch = [&](auto ph) {
+        constexpr int PH = decltype(ph)::value;
+        const int nt = c.nqtiles[PH];
+        if (nt <= 0) return;
+        int32_t* st = sync + (PH == PH_F ? 0 : QueueSyncInts(c.nnodes, c.nqueues, c.nqreds[0]));
+        if (prof) (void)hipEventRecord(prof->a, s);
+        auto go = [&](auto rt, auto kb) {
+            constexpr int RT = decltype(rt)::value, KB = decltype(kb)::value;
+            const int G = std::max(1, std::min(nt, MvqGrid<T, PH, RT, KB>(c.cus)));
+            hipLaunchKernelGGL((k_mvq<T, PH, RT, KB>), dim3(G), dim3(256), 0, s, c.qtiles[PH],
+                               c.qstart[PH], c.qreds[PH], c.qdep[PH], c.nodes, st, c.nnodes,
+                               c.nqueues, c.structs, gs, L, LT, Xp, W, P, k);
+        };
+        auto byRows = [&](auto kb) {
+            if (c.queueRows == 64)
+                go(std::integral_constant<int, 64>(), kb);
+            else if (c.queueRows == 16)
+                go(std::integral_constant<int, 16>(), kb);
+            else
+                go(std::integral_constant<int, 32>(), kb);
+        };
+        if (k <= 8)
+            byRows(std::integral_constant<int, 8>());
+        else
+            byRows(std::integral_constant<int, 16>());
+        if (prof) {
+            (void)hipEventRecord(prof->b, s);
+            (void)hipEventSynchronize(prof->b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, prof->a, prof->b);
+            prof->launches += 1;
+            prof->seconds += ms * 1e-3;
+            const double eb = sizeof(cx<T>);
+            const ClassPlan::QueuePhase& Q = plan.queue[PH];
+            prof->bytes += eb * (Q.entries + Q.vecRows * k);
+            prof->flops += 8.0 * Q.entries * k;
+        }
+    };
+    const int lowest = fused ? plan.qcut : plan.maxDepth + 1;  // first level of the fused launch
+    if (fused) fusedLaunch(std::integral_constant<int, PH_F>());
+    for (int d = std::min(plan.maxDepth, lowest - 1); d >= 0; --d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
         mv(std::integral_constant<int, PH_F>(), lv, lv.f);
     }
-    for (int d = 0; d <= plan.maxDepth; ++d) {
+    for (int d = 0; d < std::min(plan.maxDepth + 1, lowest); ++d) {
         const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
         mv(std::integral_constant<int, PH_B>(), lv, lv.b);
     }
+    if (fused) fusedLaunch(std::integral_constant<int, PH_B>());
 }
 
 template <typename T>
@@ -2170,10 +2622,11 @@ void LaunchFoldBack(const ClassDev& c, hipStream_t s)
 
 template <typename T>
 void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int ny, int nz,
-                  int vOffset, int padding, hipStream_t s)
+                  int vOffset, int padding, hipStream_t s, int32_t* sync)
 {
     hipLaunchKernelGGL((k_copy_in<T>), dim3(Blocks((int64_t)c.n * k)), dim3(kThreads), 0, s, c.n,
-                       k, c.qxy, c.qv, B, (cx<T>*)Xp, nx, ny, nz, vOffset, padding);
+                       k, c.qxy, c.qv, B, (cx<T>*)Xp, nx, ny, nz, vOffset, padding, sync,
+                       sync ? c.nsync : 0);
 }
 
 template <typename T>
@@ -2189,10 +2642,12 @@ void LaunchCopyOut(const ClassDev& c, c128* B, const void* Xp, int k, int nx, in
 
 template <typename T>
 void LaunchZeroSetPlane(const ClassDev& c, const StencilArgs& st, const double* vel,
-                        const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s)
+                        const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s,
+                        int32_t* sync)
 {
     hipLaunchKernelGGL((k_zero_set_plane<T>), dim3(Blocks((int64_t)c.n * k)), dim3(kThreads), 0, s,
-                       c.n, k, c.qxy, c.qv, st, vel, B, (cx<T>*)Xp, v2, vlLast);
+                       c.n, k, c.qxy, c.qv, st, vel, B, (cx<T>*)Xp, v2, vlLast, sync,
+                       sync ? c.nsync : 0);
 }
 
 void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c128* b, c128* y,
@@ -2281,16 +2736,16 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
     template void LaunchGemm<T>(const ClassDev&, int, int, hipStream_t);                          \
     template void LaunchInvDiag<T>(const ClassDev&, int, hipStream_t);                            \
     template void LaunchPanelSolve<T>(const ClassDev&, const ClassPlan&, int, void*, void*, void*,   \
-                                      void*, int, hipStream_t, MvProfile*, int, int);             \
+                                      void*, int, hipStream_t, MvProfile*, int, int, int32_t*);   \
     template void LaunchTransposeScale<T>(const ClassDev&, hipStream_t);                          \
     template void LaunchFoldBack<T>(const ClassDev&, hipStream_t);                                \
     template void LaunchCopyIn<T>(const ClassDev&, const c128*, void*, int, int, int, int, int,   \
-                                  int, hipStream_t);                                              \
+                                  int, hipStream_t, int32_t*);                                    \
     template void LaunchCopyOut<T>(const ClassDev&, c128*, const void*, int, int, int, int, int,  \
                                    int, int, hipStream_t, const StencilArgs*, const double*,      \
                                    int);                                                          \
     template void LaunchZeroSetPlane<T>(const ClassDev&, const StencilArgs&, const double*,       \
-                                        const c128*, void*, int, int, int, hipStream_t);
+                                        const c128*, void*, int, int, int, hipStream_t, int32_t*);
 PSP_INST(double)
 PSP_INST(float)
 #undef PSP_INST

@@ -85,7 +85,22 @@ struct ClassDev {
     void* LT = nullptr;             // npanels * fstore: per front L^T (s x (s+u), ld s)
     int cus = 0;                    // compute units of the device (persistent grids)
     int mvpGrid = 0;                // persistent solve launches: workgroup cap (0: none)
+    // fused subtree solves, per phase (F, B): queue tiles, queue starts
+    // (nqueues + 1), split row blocks, per-node dependencies {dep0, dep1,
+    // row blocks of dep0, of dep1} (-1: none)
+    QTile* qtiles[2] = {nullptr, nullptr};
+    int32_t* qstart[2] = {nullptr, nullptr};
+    QRed* qreds[2] = {nullptr, nullptr};
+    int4* qdep[2] = {nullptr, nullptr};
+    int nqueues = 0;                // 0: no fused launch for this class
+    int nqtiles[2] = {0, 0}, nqreds[2] = {0, 0};
+    int queueRows = 32;             // output rows per fused tile
+    int nsync = 0;                  // ints of fused-launch state per class (both phases)
 };
+
+// Ints of fused-launch state of one phase: per-node done counters, queue
+// heads, the exit counter and one arrival counter per split row block.
+inline int QueueSyncInts(int nnodes, int nqueues, int nreds) { return nnodes + nqueues + 1 + nreds; }
 
 
 // ---- launch wrappers (kernels.hip) ----
@@ -118,20 +133,25 @@ struct MvProfile {
 
 // Solve of one panel (batch entry b of class c) on the padded ND vector Xp
 // (n x k, k interleaved) with node work vectors W (wstore x k).
+// sync: the class's fused-launch state (cleared by the copy-in or
+// zero-set launch before the panel solve); null runs every level as its own
+// launch.
 template <typename T>
 void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xp, void* W,
                       void* P, void* G,
                       int k, hipStream_t s, MvProfile* prof = nullptr, int onlyLevel = -1,
-                      int onlyPhase = -1);
+                      int onlyPhase = -1, int32_t* sync = nullptr);
 template <typename T>
 void LaunchTransposeScale(const ClassDev& c, hipStream_t s);
 template <typename T>
 void LaunchFoldBack(const ClassDev& c, hipStream_t s);
 
 // Natural-order <-> padded panel vector.  mode 0: B = Xp, 1: B += Xp.
+// sync (nsync ints, may be null): the class's fused-launch state, cleared
+// by the same launch (the panel solve that follows starts from zero).
 template <typename T>
 void LaunchCopyIn(const ClassDev& c, const c128* B, void* Xp, int k, int nx, int ny, int nz,
-                  int vOffset, int padding, hipStream_t s);
+                  int vOffset, int padding, hipStream_t s, int32_t* sync = nullptr);
 template <typename T>
 void LaunchCopyOut(const ClassDev& c, c128* B, const void* Xp, int k, int nx, int ny, int nz,
                    int vOffset, int padding, int mode, hipStream_t s,
@@ -140,7 +160,8 @@ void LaunchCopyOut(const ClassDev& c, c128* B, const void* Xp, int k, int nx, in
 // panel's last local plane (MultiplySuperdiagonal, solve.hpp:725-733).
 template <typename T>
 void LaunchZeroSetPlane(const ClassDev& c, const StencilArgs& st, const double* vel,
-                        const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s);
+                        const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s,
+                        int32_t* sync = nullptr);
 
 // y = A x (mode 0) or y = b - A x (mode 1), natural order.
 void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c128* b,

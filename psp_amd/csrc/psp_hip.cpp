@@ -58,6 +58,7 @@ struct psp_hip_ctx {
     void *Xp = nullptr, *W = nullptr;
     void* P = nullptr;  // partial sums of split solve row blocks (64 rows x k per slot)
     void* G[3] = {nullptr, nullptr, nullptr};  // per class: the children's update rows, two slot copies
+    int32_t* S[3] = {nullptr, nullptr, nullptr};  // per class: fused-launch state (counters)
     c128 *V = nullptr, *x = nullptr, *x0 = nullptr, *w = nullptr, *b = nullptr;
     double *partD = nullptr, *normD = nullptr;
     c128 *partC = nullptr, *Hcol = nullptr, *ydev = nullptr;
@@ -217,6 +218,8 @@ std::vector<SolveTile> SolveTiles(const ClassPlan& P, const std::vector<MvTile>&
     return out;
 }
 
+int UploadQueues(psp_hip_ctx* c, int cl);
+
 // Scratch of psp_hip_precondition_noalloc, the sweep entry point that
 // allocates nothing: the sweep buffers (SweepBufferSizes) live in this
 // file-scope device array, one copy per device, held until the process
@@ -358,6 +361,7 @@ int EnsureDevice(psp_hip_ctx* c)
         D.nfold = (int)P.foldBack.size();
         std::vector<c128> s3c = ToC128(s3a);
         if ((st = Upload(c, &D.s3art, s3c))) return st;
+        if ((st = UploadQueues(c, cl))) return st;
     }
     c->devReady = true;
     return PSP_HIP_OK;
@@ -441,7 +445,7 @@ int64_t NeededSlots(const psp_hip_ctx* c)
 // children's update rows.  Vectors and slot copies end in kSolvePad + 2
 // rows that must read as zero (the solve kernels read past a tile's rows).
 struct SweepSizes {
-    size_t xp = 0, w = 0, p = 0, g[3] = {0, 0, 0};
+    size_t xp = 0, w = 0, p = 0, g[3] = {0, 0, 0}, s[3] = {0, 0, 0};
 };
 
 SweepSizes SweepBufferSizes(const psp_hip_ctx* c, int k)
@@ -460,8 +464,10 @@ SweepSizes SweepBufferSizes(const psp_hip_ctx* c, int k)
     z.w = eb * (maxw + padRows) * k;
     z.p = eb * maxs * 64 * k;
     for (int cl = 0; cl < 3; ++cl)
-        if (c->plan.present[cl])
+        if (c->plan.present[cl]) {
             z.g[cl] = eb * (2 * (size_t)c->plan.cls[cl].wstore + padRows) * k;
+            z.s[cl] = sizeof(int32_t) * (size_t)std::max(c->dev[cl].nsync, 1);
+        }
     return z;
 }
 
@@ -485,6 +491,9 @@ int EnsureWorkspace(psp_hip_ctx* c, int k, int m)
         // every solve, and the rows it never writes must read as zero.
         if ((st = Alloc(c, &c->G[cl], z.g[cl], c->wsAllocs))) return st;
         HIPCHK(c, hipMemsetAsync(c->G[cl], 0, z.g[cl], c->stream));
+        void* sp = nullptr;
+        if ((st = Alloc(c, &sp, z.s[cl], c->wsAllocs))) return st;
+        c->S[cl] = (int32_t*)sp;
     }
     if ((st = Alloc(c, &p, sizeof(c128) * N * k * m, c->wsAllocs))) return st;
     c->V = (c128*)p;
@@ -526,6 +535,44 @@ int ReUpload(psp_hip_ctx* c, X** dst, const std::vector<X>& src)
     return Upload(c, dst, src);
 }
 
+// Device copies of a class's fused subtree launches (none when the plan has
+// no cut or the option is off).
+int UploadQueues(psp_hip_ctx* c, int cl)
+{
+    const ClassPlan& P = c->plan.cls[cl];
+    ClassDev& D = c->dev[cl];
+    const bool on = P.qcut >= 0 && P.nqueues > 0;
+    for (int ph = 0; ph < 2; ++ph) {
+        const ClassPlan::QueuePhase& Q = P.queue[ph];
+        std::vector<int4> dep;
+        if (on) {
+            dep.resize(Q.dep.size());
+            for (size_t t = 0; t < Q.dep.size(); ++t)
+                dep[t] = make_int4(Q.dep[t][0], Q.dep[t][1], Q.dep[t][2], Q.dep[t][3]);
+        }
+        int st;
+        if ((st = ReUpload(c, &D.qtiles[ph], on ? Q.tiles : std::vector<QTile>{})) ||
+            (st = ReUpload(c, &D.qstart[ph], on ? Q.qstart : std::vector<int32_t>{})) ||
+            (st = ReUpload(c, &D.qreds[ph], on ? Q.reds : std::vector<QRed>{})) ||
+            (st = ReUpload(c, &D.qdep[ph], dep)))
+            return st;
+        D.nqtiles[ph] = on ? (int)Q.tiles.size() : 0;
+        D.nqreds[ph] = on ? (int)Q.reds.size() : 0;
+    }
+    D.nqueues = on ? P.nqueues : 0;
+    D.queueRows = P.queueRows;
+    D.nsync = on ? QueueSyncInts(D.nnodes, D.nqueues, D.nqreds[0]) +
+                       QueueSyncInts(D.nnodes, D.nqueues, D.nqreds[1])
+                 : 0;
+    return PSP_HIP_OK;
+}
+
+void BuildClassQueues(psp_hip_ctx* c, ClassPlan& P)
+{
+    const Plan& pl = c->plan;
+    BuildQueues(P, pl.fused ? pl.fusedQueues : 0, pl.fusedRows, pl.fusedCut, pl.fusedWgPerQueue);
+}
+
 // Rebuilds the front-solve tiles of every class from its shapes and, once
 // the device is set up, replaces the device copies; the workspace (partial
 // buffer sized by the slots) and the captured sweep graphs are dropped.
@@ -536,11 +583,12 @@ int RebuildSolveTiles(psp_hip_ctx* c)
         if (!c->plan.present[cl]) continue;
         ClassPlan& P = c->plan.cls[cl];
         BuildSolveTiles(P);
+        BuildClassQueues(c, P);
         if (!c->devReady) continue;
         ClassDev& D = c->dev[cl];
         int st;
         if ((st = ReUpload(c, &D.mv, SolveTiles(P, P.mv))) ||
-            (st = ReUpload(c, &D.mvred, SolveTiles(P, P.mvred))))
+            (st = ReUpload(c, &D.mvred, SolveTiles(P, P.mvred))) || (st = UploadQueues(c, cl)))
             return st;
     }
     FreeWorkspace(c);
@@ -632,9 +680,10 @@ void PanelForward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = nullp
     const ClassDev& D = c->dev[cl];
     hipStream_t s = c->stream;
     const StencilArgs st = Stencil(c);
-    LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s);
+    int32_t* sync = D.nqueues > 0 ? c->S[cl] : nullptr;
+    LaunchCopyIn<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), s, sync);
     LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, c->G[cl], k, s,
-                        prof);
+                        prof, -1, -1, sync);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 0, s,
                      couple >= 0 ? &st : nullptr, c->vel, couple);
 }
@@ -652,9 +701,10 @@ void PanelBackward(psp_hip_ctx* c, c128* B, int k, int p, MvProfile* prof = null
     // Xp = 0 except the superdiagonal coupling on the panel's last plane
     const int v2 = g.panelV(p) + g.depth(p) - 1;
     const int vlLast = g.padding(p) + g.depth(p) - 1;
-    LaunchZeroSetPlane<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s);
+    int32_t* sync = D.nqueues > 0 ? c->S[cl] : nullptr;
+    LaunchZeroSetPlane<T>(D, Stencil(c), c->vel, B, c->Xp, k, v2, vlLast, s, sync);
     LaunchPanelSolve<T>(D, c->plan.cls[cl], c->panelBatch[p], c->Xp, c->W, c->P, c->G[cl], k, s,
-                        prof);
+                        prof, -1, -1, sync);
     LaunchCopyOut<T>(D, B, c->Xp, k, g.nx, g.ny, g.nz, g.vOffset(p), g.padding(p), 1, s);
 }
 
@@ -993,6 +1043,7 @@ int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
         auto timeShape = [&](int d, int ph, SolveShape sh, double* t) -> int {
             P.shapes[d][ph] = sh;
             BuildSolveTiles(P);
+            BuildClassQueues(c, P);  // keeps the partial slots of the fused launches
             int st;
             if ((st = ReUpload(c, &D.mv, SolveTiles(P, P.mv))) ||
                 (st = ReUpload(c, &D.mvred, SolveTiles(P, P.mvred))) ||
@@ -1146,6 +1197,30 @@ int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
         c->mvpGrid = value;
         for (auto& D : c->dev) D.mvpGrid = value;
         return PSP_HIP_OK;
+    }
+    if (option == PSP_HIP_OPT_SOLVE_FUSED || option == PSP_HIP_OPT_SOLVE_FUSED_QUEUES ||
+        option == PSP_HIP_OPT_SOLVE_FUSED_ROWS || option == PSP_HIP_OPT_SOLVE_FUSED_CUT ||
+        option == PSP_HIP_OPT_SOLVE_FUSED_WG) {
+        if (option == PSP_HIP_OPT_SOLVE_FUSED) {
+            c->plan.fused = value != 0;
+        } else if (option == PSP_HIP_OPT_SOLVE_FUSED_QUEUES) {
+            if (value < 1 || value > 64)
+                return Fail(c, PSP_HIP_ERR_INVALID, "fused queues out of range [1, 64]");
+            c->plan.fusedQueues = value;
+        } else if (option == PSP_HIP_OPT_SOLVE_FUSED_ROWS) {
+            if (value != 16 && value != 32 && value != 64)
+                return Fail(c, PSP_HIP_ERR_INVALID, "fused tile rows must be 16, 32 or 64");
+            c->plan.fusedRows = value;
+        } else if (option == PSP_HIP_OPT_SOLVE_FUSED_WG) {
+            if (value < 1 || value > 4096)
+                return Fail(c, PSP_HIP_ERR_INVALID, "fused workgroups per queue out of range");
+            c->plan.fusedWgPerQueue = value;
+        } else {
+            if (value < -1 || value > 64)
+                return Fail(c, PSP_HIP_ERR_INVALID, "fused cut depth out of range [-1, 64]");
+            c->plan.fusedCut = value;
+        }
+        return RebuildSolveTiles(c);
     }
     if (option == PSP_HIP_OPT_SOLVE_MIN_TILES || option == PSP_HIP_OPT_SOLVE_MAX_SPLIT ||
         option == PSP_HIP_OPT_SOLVE_WIDE_WG || option == PSP_HIP_OPT_SOLVE_PERSIST) {
@@ -1304,7 +1379,8 @@ int psp_hip_precondition_noalloc(psp_hip_ctx* c, void* B, int k)
     if (!c->scratch) return Fail(c, PSP_HIP_ERR_STATE, "device scratch not set up");
     const SweepSizes z = SweepBufferSizes(c, k);
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t need = up(z.xp) + up(z.w) + up(z.p) + up(z.g[0]) + up(z.g[1]) + up(z.g[2]);
+    const size_t need = up(z.xp) + up(z.w) + up(z.p) + up(z.g[0]) + up(z.g[1]) + up(z.g[2]) +
+                        up(z.s[0]) + up(z.s[1]) + up(z.s[2]);
     if (need > kScratchBytes)
         return Fail(c, PSP_HIP_ERR_INVALID,
                     "precondition_noalloc: the sweep needs " + std::to_string(need) +
@@ -1315,6 +1391,7 @@ int psp_hip_precondition_noalloc(psp_hip_ctx* c, void* B, int k)
     // fill kernel clears the whole region (no memset node if captured)
     LaunchFillZero(base, (int64_t)need, c->stream);
     void* keep[6] = {c->Xp, c->W, c->P, c->G[0], c->G[1], c->G[2]};
+    int32_t* keepS[3] = {c->S[0], c->S[1], c->S[2]};
     size_t off = 0;
     auto take = [&](size_t b) {
         void* q = b ? base + off : nullptr;
@@ -1325,11 +1402,13 @@ int psp_hip_precondition_noalloc(psp_hip_ctx* c, void* B, int k)
     c->W = take(z.w);
     c->P = take(z.p);
     for (int cl = 0; cl < 3; ++cl) c->G[cl] = take(z.g[cl]);
+    for (int cl = 0; cl < 3; ++cl) c->S[cl] = (int32_t*)take(z.s[cl]);
     SweepEager(c, (c128*)B, k);
     c->Xp = keep[0];
     c->W = keep[1];
     c->P = keep[2];
     for (int cl = 0; cl < 3; ++cl) c->G[cl] = keep[3 + cl];
+    for (int cl = 0; cl < 3; ++cl) c->S[cl] = keepS[cl];
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return PSP_HIP_OK;

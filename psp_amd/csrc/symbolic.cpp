@@ -2,6 +2,7 @@
 #include "symbolic.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <stack>
 
@@ -531,6 +532,8 @@ void Schedule(const Plan& plan, ClassPlan& C)
             }
         DefaultSolveShapes(C, plan.minTiles, plan.maxSplit, plan.wideWg, plan.persist);
         BuildSolveTiles(C);
+        BuildQueues(C, plan.fused ? plan.fusedQueues : 0, plan.fusedRows, plan.fusedCut,
+                    plan.fusedWgPerQueue);
         // Solve layout: LT[:, :s] = Linv^T D^-1 (32x32 blocks of L_TL),
         // LT[:, s:] = (-L_BL Linv)^T by one batched GEMM, then L_BL and the
         // diagonal of L from LT (an item with i0 == 0 per column block also
@@ -728,6 +731,139 @@ void BuildSolveTiles(ClassPlan& C)
         account(L, 0, L.f);
         account(L, 1, L.b);
     }
+}
+
+void BuildQueues(ClassPlan& C, int nqueues, int rows, int cut, int wgPerQueue)
+{
+    C.qcut = -1;
+    C.nqueues = 0;
+    C.queueRows = rows;
+    for (auto& Q : C.queue) Q = ClassPlan::QueuePhase{};
+    if (nqueues <= 0 || C.nodes.empty()) return;
+    // cut: the first depth with nqueues nodes (or the requested one)
+    int qc = -1;
+    for (int d = 0; d <= C.maxDepth; ++d)
+        if ((int)C.levels[d].size() >= std::min(nqueues, kQueues)) {
+            qc = d;
+            break;
+        }
+    if (cut >= 0) qc = qc < 0 ? -1 : std::max(qc, cut);
+    if (qc < 0 || qc > C.maxDepth) return;
+    const int nn = (int)C.nodes.size();
+    // row blocks of a node per phase, and the node's factor entries (both
+    // phases read the same count)
+    auto rb = [&](int ph, int t) -> int {
+        const Node& n = C.nodes[t];
+        return SolveWanted(ph, n) ? (int)CeilDiv(SolveRowsOf(ph, n), rows) : 0;
+    };
+    std::vector<double> sub(nn, 0.0);  // subtree entries (post-order: children first)
+    for (int t = 0; t < nn; ++t) {
+        const Node& n = C.nodes[t];
+        sub[t] += (double)n.size * (n.size + 1) / 2 + (double)n.usize * n.size;
+        if (n.parent >= 0) sub[n.parent] += sub[t];
+    }
+    // the cut nodes' subtrees over the queues, largest first to the least
+    // loaded queue
+    std::vector<int> cutNodes = C.levels[qc];
+    std::sort(cutNodes.begin(), cutNodes.end(),
+              [&](int a, int b) { return sub[a] != sub[b] ? sub[a] > sub[b] : a < b; });
+    std::vector<double> load(nqueues, 0.0);
+    std::vector<int> queueOf(nn, -1);
+    for (int t : cutNodes) {
+        int best = 0;
+        for (int q = 1; q < nqueues; ++q)
+            if (load[q] < load[best]) best = q;
+        load[best] += sub[t];
+        queueOf[t] = best;
+    }
+    // post-order: a parent comes after its children, so walk the nodes from
+    // the root down to give every fused node its cut ancestor's queue
+    for (int t = nn - 1; t >= 0; --t) {
+        const Node& n = C.nodes[t];
+        if (n.depth > qc && n.parent >= 0) queueOf[t] = queueOf[n.parent];
+    }
+    constexpr int kMinSlice = 64;
+    int slots = 0;
+    for (int ph = 0; ph < 2; ++ph) {
+        ClassPlan::QueuePhase& Q = C.queue[ph];
+        Q.dep.assign(nn, {-1, -1, 0, 0});
+        for (int t = 0; t < nn; ++t) {
+            const Node& n = C.nodes[t];
+            if (n.depth < qc) continue;
+            auto& dp = Q.dep[t];
+            if (ph == 0) {
+                int j = 0;
+                for (int c = 0; c < n.nchild; ++c) {
+                    const int ch = n.child[c];
+                    if (ch >= 0 && rb(0, ch) > 0) {
+                        dp[j] = ch;
+                        dp[2 + j] = rb(0, ch);
+                        ++j;
+                    }
+                }
+            } else {
+                int a = n.parent;
+                while (a >= 0 && rb(1, a) == 0) a = C.nodes[a].parent;
+                if (a >= 0 && C.nodes[a].depth >= qc) {
+                    dp[0] = a;
+                    dp[2] = rb(1, a);
+                }
+            }
+        }
+        // inner slices per depth: enough tiles for the queue's workgroups
+        std::vector<int> split(C.maxDepth + 1, 1);
+        for (int d = qc; d <= C.maxDepth; ++d) {
+            int64_t r = 0;
+            for (int t : C.levels[d]) r += rb(ph, t);
+            const double perQueue = (double)r / nqueues;
+            if (perQueue > 0 && perQueue < wgPerQueue)
+                split[d] = std::min(8, (int)std::ceil(wgPerQueue / perQueue));
+        }
+        Q.qstart.assign(nqueues + 1, 0);
+        for (int q = 0; q < nqueues; ++q) {
+            Q.qstart[q] = (int)Q.tiles.size();
+            for (int i = 0; i <= C.maxDepth - qc; ++i) {
+                const int d = ph == 0 ? C.maxDepth - i : qc + i;
+                for (int t : C.levels[d]) {
+                    if (queueOf[t] != q) continue;
+                    const Node& n = C.nodes[t];
+                    if (!SolveWanted(ph, n)) continue;
+                    const int nr = SolveRowsOf(ph, n);
+                    for (int r0 = 0; r0 < nr; r0 += rows) {
+                        const int lo = SolveLo(ph, r0);
+                        const int hi = std::max(lo, SolveHi(ph, n, r0, rows));
+                        const int len = hi - lo;
+                        int per = len, nsl = 1;
+                        if (split[d] > 1 && len >= 2 * kMinSlice) {
+                            per = std::max(kMinSlice, (int)CeilDiv(CeilDiv(len, split[d]), 16) * 16);
+                            nsl = (int)CeilDiv(len, per);
+                        }
+                        QTile m{t, r0, std::min(rows, nr - r0), lo, hi, -1, -1, 0};
+                        if (nsl > 1) {
+                            m.red = (int)Q.reds.size();
+                            Q.reds.push_back(QRed{t, r0, m.nrows, slots, nsl, {0, 0, 0}});
+                        }
+                        for (int j = 0; j < nsl; ++j) {
+                            if (nsl > 1) {
+                                m.col0 = lo + j * per;
+                                m.col1 = std::min(hi, lo + (j + 1) * per);
+                                m.slot = slots++;
+                            }
+                            Q.tiles.push_back(m);
+                        }
+                    }
+                }
+            }
+        }
+        Q.qstart[nqueues] = (int)Q.tiles.size();
+        for (int d = qc; d <= C.maxDepth; ++d) {
+            Q.entries += C.solveLevels[d].entries[ph];
+            Q.vecRows += C.solveLevels[d].vecRows[ph];
+        }
+    }
+    C.maxSlots = std::max(C.maxSlots, slots);
+    C.qcut = qc;
+    C.nqueues = nqueues;
 }
 
 int BuildPlan(const psp_hip_desc& d, Plan& plan, std::string& msg)

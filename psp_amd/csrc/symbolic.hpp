@@ -114,6 +114,21 @@ struct MvTile {
     int32_t slot;
 };
 
+// Fused subtree solves: one persistent launch per phase runs every level of
+// a panel's tree from the cut depth down, the subtrees under the cut split
+// over kQueues queues (one per XCD of the device, each queue's tiles in
+// dependency order).  A queue tile names its node (the node's fields come
+// from the per-node records); red >= 0: the tile is one inner slice
+// (partial sums in slot `slot`) of the split row block QRed[red].
+constexpr int kQueues = 8;
+struct QTile {
+    int32_t node, row0, nrows, col0, col1, slot, red, pad_;
+};
+struct QRed {
+    int32_t node, row0, nrows, slot0, nslots, pad_[3];
+};
+static_assert(sizeof(QTile) == 32 && sizeof(QRed) == 32, "queue records");
+
 // Shape of the tiles of one (level, phase) of the front solves.
 struct SolveShape {
     int rows = 64;  // output rows per tile: 64, 32 or 16
@@ -214,6 +229,25 @@ struct ClassPlan {
     std::vector<SolveLevel> solveLevels;  // index = depth
     std::vector<TransItem> trans;
 
+    // Fused subtree solves (qcut < 0: none; every level is its own launch).
+    // Per phase: the tiles of queue q are tiles[qstart[q] .. qstart[q+1]),
+    // forward deepest level first, backward the cut level first; dep[t] =
+    // {dep0, dep1, row blocks of dep0, of dep1} names the nodes whose row
+    // blocks must be done before node t's tiles read their inputs (forward:
+    // its children; backward: its nearest ancestor with pivots, when that
+    // is inside the fused levels), -1 for none.
+    int qcut = -1;
+    int queueRows = 32;  // output rows per queue tile (64, 32 or 16)
+    int nqueues = 0;
+    struct QueuePhase {
+        std::vector<QTile> tiles;
+        std::vector<int32_t> qstart;
+        std::vector<QRed> reds;
+        std::vector<std::array<int32_t, 4>> dep;
+        double entries = 0, vecRows = 0;  // algorithmic work of the launch
+    };
+    QueuePhase queue[2];
+
     // Algorithmic work per panel.
     double flops = 0, schurFlops = 0, solveFlopsPerRhs = 0;
     double factorEntries = 0;
@@ -229,6 +263,14 @@ struct Plan {
     int maxSplit = 8;      // inner slices per row block at most (default shapes)
     bool wideWg = false;   // default shapes: 16-wave workgroups instead of slices
     bool persist = true;   // default shapes: persistent pipelined workgroups
+    bool fused = false;    // fused subtree launches (8 <= k <= 16)
+    int fusedCut = -1;     // cut depth of the fused launches (-1: the first with
+                           // at least kQueues nodes)
+    int fusedRows = 32;    // output rows per queue tile
+    int fusedQueues = kQueues;  // queues (more than the device's XCDs leaves
+                                // queues that the last workgroup runs alone)
+    int fusedWgPerQueue = 64;   // workgroups a queue expects (inner slices are
+                                // cut so a level has about that many tiles)
 };
 
 // Builds everything; returns PSP_HIP_OK or an error code with `msg` set.
@@ -242,5 +284,10 @@ void DefaultSolveShapes(ClassPlan& C, int minTiles, int maxSplit, bool wideWg = 
 // (Re)builds the solve tiles, reduce items and their accounting from
 // C.shapes.
 void BuildSolveTiles(ClassPlan& C);
+// (Re)builds the fused subtree launches of a class (after BuildSolveTiles:
+// uses its per-level accounting and raises maxSlots to cover the queues'
+// partial slots).  nqueues <= 0 or a tree with no level of nqueues nodes
+// leaves qcut = -1.
+void BuildQueues(ClassPlan& C, int nqueues, int rows, int cut, int wgPerQueue);
 
 }  // namespace psp_amd

@@ -1,0 +1,179 @@
+// Host-side check of the fused subtree work lists (BuildQueues in
+// psp_amd/csrc/symbolic.cpp) for the grids given on the command line:
+//   fused_plan_check nx ny nz [queues rows cut wgPerQueue] ...
+// For every panel class and phase it checks that
+//  - every node at or below the cut has each of its row blocks in exactly
+//    one queue, as one tile or as slices that partition the row block's
+//    inner range, with distinct partial slots below maxSlots;
+//  - every index a kernel reads (node, slot, reduce record, queue range)
+//    is inside its array;
+//  - a tile's dependencies are the nodes the method needs (forward: the
+//    children with rows; backward: the nearest ancestor with pivots, when it
+//    is at or below the cut), each with its number of row blocks, and every
+//    row block of a dependency completes earlier in the same queue;
+//  - the per-level accounting covers the same levels.
+// Prints one line per class and exits nonzero on the first violation.
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "symbolic.hpp"
+
+using namespace psp_amd;
+
+static int fail(const char* what, int cls, int ph, int a = 0, int b = 0)
+{
+    std::fprintf(stderr, "FAIL class %d phase %d: %s (%d, %d)\n", cls, ph, what, a, b);
+    return 1;
+}
+
+static int rowsOf(int ph, const Node& n) { return ph == 0 ? n.size + n.usize : n.size; }
+
+static int check(const ClassPlan& C, int cls)
+{
+    if (C.qcut < 0) {
+        std::printf("class %d: no fused launch\n", cls);
+        return 0;
+    }
+    const int nn = (int)C.nodes.size(), nq = C.nqueues, R = C.queueRows;
+    for (int ph = 0; ph < 2; ++ph) {
+        const auto& Q = C.queue[ph];
+        if ((int)Q.qstart.size() != nq + 1 || Q.qstart[0] != 0 || Q.qstart[nq] != (int)Q.tiles.size())
+            return fail("queue starts", cls, ph);
+        if ((int)Q.dep.size() != nn) return fail("dependency table size", cls, ph);
+        std::vector<int> queueOfNode(nn, -1);
+        std::map<std::pair<int, int>, int> blockDonePos;  // (node, row0) -> tile index completing it
+        std::vector<int> seenSlot(C.maxSlots, 0);
+        std::map<int, std::vector<std::pair<int, int>>> sliceCols;  // red -> col ranges
+        for (int q = 0; q < nq; ++q) {
+            if (Q.qstart[q] > Q.qstart[q + 1]) return fail("queue order", cls, ph, q);
+            std::map<int, int> redArrived;
+            for (int i = Q.qstart[q]; i < Q.qstart[q + 1]; ++i) {
+                const QTile& t = Q.tiles[i];
+                if (t.node < 0 || t.node >= nn) return fail("node index", cls, ph, i, t.node);
+                const Node& n = C.nodes[t.node];
+                if (n.depth < C.qcut) return fail("node above the cut", cls, ph, t.node);
+                if (queueOfNode[t.node] >= 0 && queueOfNode[t.node] != q)
+                    return fail("node in two queues", cls, ph, t.node);
+                queueOfNode[t.node] = q;
+                if (t.row0 < 0 || t.row0 % R || t.nrows < 1 || t.nrows > R ||
+                    t.row0 + t.nrows > rowsOf(ph, n))
+                    return fail("row block", cls, ph, i, t.row0);
+                const int lo = ph == 0 ? 0 : t.row0;
+                const int hi = ph == 0 ? std::min(n.size, t.row0 + R) : n.size + n.usize;
+                if (t.col0 < lo || t.col1 > std::max(lo, hi) || t.col0 > t.col1)
+                    return fail("inner range", cls, ph, i, t.col0);
+                // dependencies come first in this queue
+                const auto& d = Q.dep[t.node];
+                for (int j = 0; j < 2; ++j) {
+                    if (d[j] < 0) continue;
+                    if (d[j] >= nn) return fail("dependency index", cls, ph, t.node, d[j]);
+                    int blocks = 0;
+                    for (int r0 = 0; r0 < rowsOf(ph, C.nodes[d[j]]); r0 += R) {
+                        auto it = blockDonePos.find({d[j], r0});
+                        if (it == blockDonePos.end() || it->second >= i)
+                            return fail("dependency not done earlier in the queue", cls, ph,
+                                        t.node, d[j]);
+                        if (queueOfNode[d[j]] != q) return fail("dependency in another queue", cls, ph);
+                        ++blocks;
+                    }
+                    if (blocks != d[2 + j]) return fail("dependency row-block count", cls, ph, t.node);
+                }
+                if (t.red < 0) {
+                    if (t.slot != -1) return fail("slot without reduce", cls, ph, i);
+                    if (t.col0 != lo || t.col1 != std::max(lo, hi)) return fail("unsplit range", cls, ph, i);
+                    if (!blockDonePos.emplace(std::make_pair(t.node, t.row0), i).second)
+                        return fail("row block twice", cls, ph, t.node, t.row0);
+                } else {
+                    if (t.red >= (int)Q.reds.size()) return fail("reduce index", cls, ph, i, t.red);
+                    const QRed& r = Q.reds[t.red];
+                    if (r.node != t.node || r.row0 != t.row0 || r.nrows != t.nrows)
+                        return fail("reduce record", cls, ph, i);
+                    if (t.slot < r.slot0 || t.slot >= r.slot0 + r.nslots || t.slot >= C.maxSlots)
+                        return fail("slot", cls, ph, i, t.slot);
+                    if (seenSlot[t.slot]++) return fail("slot twice", cls, ph, t.slot);
+                    sliceCols[t.red].push_back({t.col0, t.col1});
+                    if (++redArrived[t.red] == r.nslots)
+                        if (!blockDonePos.emplace(std::make_pair(t.node, t.row0), i).second)
+                            return fail("row block twice", cls, ph, t.node, t.row0);
+                }
+            }
+        }
+        for (auto& kv : sliceCols) {
+            const QRed& r = Q.reds[kv.first];
+            if ((int)kv.second.size() != r.nslots) return fail("slices per reduce", cls, ph, kv.first);
+            const Node& n = C.nodes[r.node];
+            const int lo = ph == 0 ? 0 : r.row0;
+            const int hi = ph == 0 ? std::min(n.size, r.row0 + R) : n.size + n.usize;
+            int at = lo;
+            for (auto& c : kv.second) {
+                if (c.first != at) return fail("slices do not tile the range", cls, ph, kv.first);
+                at = c.second;
+            }
+            if (at != hi) return fail("slices end", cls, ph, kv.first);
+        }
+        // every row block of every fused node, and the expected dependencies
+        for (int t = 0; t < nn; ++t) {
+            const Node& n = C.nodes[t];
+            if (n.depth < C.qcut) continue;
+            const bool wanted = ph == 0 ? n.size + n.usize > 0 : n.size > 0;
+            for (int r0 = 0; wanted && r0 < rowsOf(ph, n); r0 += R)
+                if (!blockDonePos.count({t, r0})) return fail("row block missing", cls, ph, t, r0);
+            std::set<int> want;
+            if (ph == 0) {
+                for (int c = 0; c < n.nchild; ++c)
+                    if (n.child[c] >= 0 && rowsOf(0, C.nodes[n.child[c]]) > 0) want.insert(n.child[c]);
+            } else {
+                int a = n.parent;
+                while (a >= 0 && C.nodes[a].size == 0) a = C.nodes[a].parent;
+                if (a >= 0 && C.nodes[a].depth >= C.qcut) want.insert(a);
+            }
+            std::set<int> got;
+            for (int j = 0; j < 2; ++j)
+                if (Q.dep[t][j] >= 0) got.insert(Q.dep[t][j]);
+            if (got != want) return fail("dependency set", cls, ph, t);
+        }
+    }
+    std::printf("class %d: cut %d, %d queues, %zu/%zu tiles, %zu/%zu split row blocks, %d slots\n",
+                cls, C.qcut, nq, C.queue[0].tiles.size(), C.queue[1].tiles.size(),
+                C.queue[0].reds.size(), C.queue[1].reds.size(), C.maxSlots);
+    return 0;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s nx ny nz [queues rows cut wgPerQueue]\n", argv[0]);
+        return 2;
+    }
+    psp_hip_desc d{};
+    d.nx = std::atoi(argv[1]);
+    d.ny = std::atoi(argv[2]);
+    d.nz = std::atoi(argv[3]);
+    const double h = 1.0 / (std::max(d.nx, std::max(d.ny, d.nz)) + 1);
+    d.wx = (d.nx + 1) * h;
+    d.wy = (d.ny + 1) * h;
+    d.wz = (d.nz + 1) * h;
+    d.omega = 30.0;
+    d.bx = d.by = d.bz = 5;
+    d.sigmax = d.sigmay = d.sigmaz = 1.5;
+    d.damping = 7.0;
+    d.planesPerPanel = 4;
+    d.cutoff = 12;
+    Plan plan;
+    if (argc > 4) plan.fusedQueues = std::atoi(argv[4]);
+    if (argc > 5) plan.fusedRows = std::atoi(argv[5]);
+    if (argc > 6) plan.fusedCut = std::atoi(argv[6]);
+    if (argc > 7) plan.fusedWgPerQueue = std::atoi(argv[7]);
+    std::string msg;
+    if (BuildPlan(d, plan, msg) != PSP_HIP_OK) {
+        std::fprintf(stderr, "plan: %s\n", msg.c_str());
+        return 2;
+    }
+    for (int c = 0; c < 3; ++c)
+        if (plan.present[c] && check(plan.cls[c], c)) return 1;
+    return 0;
+}
