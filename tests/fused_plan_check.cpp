@@ -164,6 +164,7 @@ int main(int argc, char** argv)
     d.planesPerPanel = 4;
     d.cutoff = 12;
     Plan plan;
+    plan.fused = true;
     if (argc > 4) plan.fusedQueues = std::atoi(argv[4]);
     if (argc > 5) plan.fusedRows = std::atoi(argv[5]);
     if (argc > 6) plan.fusedCut = std::atoi(argv[6]);

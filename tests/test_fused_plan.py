@@ -31,4 +31,4 @@ def checker(tmp_path_factory):
 def test_fused_work_lists(checker, args):
     out = subprocess.run([checker] + args.split(), capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert "class 0:" in out.stdout
+    assert "class 0: cut" in out.stdout
