@@ -262,6 +262,16 @@ def main():
                          "100^3 grid (0: skip)")
     ap.add_argument("--wide-steps", type=int, default=2,
                     help="timed solves of the wide block per precision")
+    ap.add_argument("--fused", type=int, default=0,
+                    help="1: run the tree levels below the cut depth as one fused launch per "
+                         "direction (8 <= k <= 16); 0: one launch per level")
+    ap.add_argument("--fused-wg", type=int, default=0,
+                    help="workgroups a fused queue expects (0: the library's default)")
+    ap.add_argument("--fused-rows", type=int, default=0,
+                    help="output rows per fused tile, 16, 32 or 64 (0: the library's default)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="bumps100: also run the oracle's whole Solve on the host cores (a few "
+                         "minutes) and report it beside the sampled estimate")
     ap.add_argument("--calibrate", action="store_true",
                     help="copy a 1 GiB device buffer once (a known-size kernel for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -313,6 +323,9 @@ def main():
         src = Bsrc if src is None else src
         kk = src.shape[1]
         h = psp_amd.DistHelmholtz(disc, precision=precision)
+        if args.fused:
+            h.set_solve_tiling(fused=1, fused_wg=args.fused_wg or None,
+                               fused_rows=args.fused_rows or None)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rep = h.Initialize(vel)
@@ -433,6 +446,13 @@ def main():
         cpu = cpu_full(wl, args)
     elif args.workload == "bumps100":
         cpu = cpu_baseline(wl, out, args)
+        if args.cpu_full:
+            full = cpu_full(wl, args)
+            cpu["full_run"] = {key: full[key] for key in ("value", "iterations", "factor_s",
+                                                          "solve_s")}
+            cpu["full_run"]["estimate_over_full"] = cpu["solve_s_est"] / full["solve_s"]
+            log(f"[cpu] estimate {cpu['solve_s_est']:.1f}s against the whole solve "
+                f"{full['solve_s']:.1f}s ({full['iterations']} its)")
     else:
         cpu = None  # the CPU restatement needs hours here; see the 100^3 line
 
@@ -451,7 +471,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": [disc.nx, disc.ny, disc.nz], "rhs": k,
                    "gmres_m": args.m, "rel_tol": args.tol, "omega": disc.omega,
-                   "sigma": disc.sigmax, "max_its": args.max_its},
+                   "sigma": disc.sigmax, "max_its": args.max_its,
+                   "sweep_launches": "fused below the cut" if args.fused else "one per level"},
         "status": "ok" if value is not None else
                   "not converged: value is null (a timed solve ended above rel_tol)",
         "roofline": roofline,
