@@ -267,6 +267,9 @@ def main():
                          "direction (8 <= k <= 16); 0: one launch per level")
     ap.add_argument("--fused-wg", type=int, default=0,
                     help="workgroups a fused queue expects (0: the library's default)")
+    ap.add_argument("--xcd-reduce", type=int, default=1,
+                    help="0: split row blocks summed by a reduce launch instead of by the "
+                         "product launch's last slice")
     ap.add_argument("--fused-rows", type=int, default=0,
                     help="output rows per fused tile, 16, 32 or 64 (0: the library's default)")
     ap.add_argument("--cpu-full", action="store_true",
@@ -323,6 +326,8 @@ def main():
         src = Bsrc if src is None else src
         kk = src.shape[1]
         h = psp_amd.DistHelmholtz(disc, precision=precision)
+        if not args.xcd_reduce:
+            h.set_solve_tiling(xcd_reduce=0)
         if args.fused:
             h.set_solve_tiling(fused=1, fused_wg=args.fused_wg or None,
                                fused_rows=args.fused_rows or None)

@@ -79,10 +79,17 @@ enum {
                                           or 64 */
     PSP_HIP_OPT_SOLVE_FUSED_CUT = 10,  /* cut depth of the fused launches; -1 (default):
                                           the first depth with 8 nodes */
-    PSP_HIP_OPT_SOLVE_FUSED_WG = 11    /* workgroups a fused queue expects (default 64):
+    PSP_HIP_OPT_SOLVE_FUSED_WG = 11,   /* workgroups a fused queue expects (default 64):
                                           near the cut, row blocks are cut into inner
                                           slices until a level has about that many
                                           tiles per queue; 1: no slices */
+    PSP_HIP_OPT_SOLVE_XCD_REDUCE = 12  /* 1 (default): the inner slices of a row block
+                                          run on one XCD and the last to finish sums
+                                          them in the product launch (one-workgroup-
+                                          per-tile levels, 8 <= k < 32), when the
+                                          device deals workgroups to its XCDs
+                                          round-robin (checked at set-up); 0: a
+                                          reduce launch follows the product launch */
 };
 
 /* Flags for psp_hip_solve. */

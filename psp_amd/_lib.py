@@ -27,6 +27,7 @@ OPT_SOLVE_FUSED_QUEUES = 8
 OPT_SOLVE_FUSED_ROWS = 9
 OPT_SOLVE_FUSED_CUT = 10
 OPT_SOLVE_FUSED_WG = 11
+OPT_SOLVE_XCD_REDUCE = 12
 
 # Every entry point include/psp_hip.h declares.
 EXPORTS = (

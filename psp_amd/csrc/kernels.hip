@@ -535,6 +535,7 @@ __global__ __launch_bounds__(kMvThreads) void k_mv(const SolveTile* tiles, int s
     constexpr int KC = 8, NW = kMvThreads / 64, CPW = 64 / RT, STRIDE = CPW * NW;
     constexpr int NI = RT == 16 ? 16 : 32, CC = STRIDE * NI;
     const SolveTile tl = tiles[start + blockIdx.x];
+    if (tl.nrows <= 0) return;  // a hole of the XCD placement
     const NodeDev nd = tl.nd;
     const int s = nd.s, fs = nd.s + nd.u;
     const cx<T>* Lf = L + nd.l;
@@ -682,6 +683,71 @@ struct Mfma<double> {
 #define PSP_MVM_OCCUPANCY
 #endif
 
+// --------------------------------------------------------------------------
+// Hand-offs between workgroups of one XCD.  Workgroups are dealt to the 8
+// XCDs round-robin (blockIdx % 8 == XCC_ID, checked at device set-up), and
+// the XCDs' L2 caches are not coherent with each other, so a hand-off that
+// stays on one XCD needs no L2 write-back: the producer waits for its
+// stores and raises a device-scope counter, the consumer sees the counter
+// and reads the data with loads that miss the L1 (L2Buf).
+// --------------------------------------------------------------------------
+__device__ __forceinline__ int XccId()
+{
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);  // HW_REG_XCC_ID, bits 3:0
+}
+
+__device__ __forceinline__ int LoadAgent(const int32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int AddAgent(int32_t* p, int v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// This wave's stores (and loads) have completed: s_waitcnt vmcnt(0).
+__device__ __forceinline__ void WaitVmem() { __builtin_amdgcn_s_waitcnt(0xF70); }
+
+// Loads of vectors that another workgroup of the XCD wrote during this
+// launch: buffer loads with the device-scope cache policy (sc1), which miss
+// the CU's L1 and read the L2, where the producer's stores are.  They are
+// ordinary loads to the compiler (scheduled and batched like any other).
+// Measured on MI355X: with sc0 loads (workgroup scope), or with an L1
+// invalidate after the wait, results went wrong whenever a vector row was
+// not a whole number of 128-byte lines (a line then holds rows of two nodes,
+// and the L1 can keep or regain a copy read before the other node was
+// written); with sc1 loads every case is exact, and so are full
+// device-scope release/acquire fences, which also write back and
+// invalidate the L2.  A buffer spans at most 4 GiB (the host checks).
+template <typename T>
+struct L2Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ explicit L2Buf(const void* base)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000))
+    {
+    }
+    __device__ cx<T> operator[](int64_t i) const
+    {
+        constexpr int kSc1 = 16;  // cache policy: SC1 set (device scope)
+        const uint32_t off = (uint32_t)(i * (int64_t)sizeof(cx<T>));
+        cx<T> v;
+        if constexpr (sizeof(T) == 4) {
+            const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kSc1);
+            __builtin_memcpy(&v, &x, 8);
+        } else {
+            const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kSc1);
+            __builtin_memcpy(&v, &x, 16);
+        }
+        return v;
+    }
+};
+
+__global__ void k_xcc_probe(int32_t* out)
+{
+    if (threadIdx.x == 0) out[blockIdx.x] = XccId();
+}
+
 // Matrix-core path (k >= 8).  One workgroup per tile; wave w owns rows
 // 16 (w / WK) .. +16 of the tile and every WK-th 4-column step of its inner
 // range.  The inner range streams through LDS in chunks of CC columns (all
@@ -692,12 +758,22 @@ struct Mfma<double> {
 // for single-pass blocks the gathered bottom rows of F) are issued right
 // after the tile record arrives, so a workgroup waits for few memory round
 // trips.
+//
+// Inner slices of a split row block (slot >= 0) write partial sums to their
+// slots.  With redc given, the host has placed the slices of each row block
+// at launch positions that are equal mod 8, so they all run on one XCD: each
+// slice waits for its partial stores and counts itself in redc[red], and the
+// last to arrive sums the block's slots in slot order (the sums of
+// k_mv_reduce, bit for bit), finishes the rows and clears the counter.
+// Without redc a reduce launch follows.  Tiles with no rows are holes in that
+// placement and exit at once.
 template <typename T, int PH, int RT, int KB, int NWV>
 __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveTile* tiles, int start,
                                                      const int32_t* structs,
                                                      Slots<T> gs, const cx<T>* L,
                                                      const cx<T>* LT, cx<T>* Xp, cx<T>* W,
-                                                     cx<T>* P, int k)
+                                                     cx<T>* P, int k, int32_t* redc,
+                                                     const SolveTile* reds)
 {
     typedef typename Mfma<T>::acc acc_t;
     constexpr int NT = 64 * NWV, WR = RT / 16, WK = NWV / WR, NB = KB / 8;
@@ -712,11 +788,13 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
     static_assert(WK == 1 || 2 * CC * KB >= WR * (WK / 2) * NB * 256,
                   "reduction scratch must fit in vs");
     const SolveTile tl = tiles[start + blockIdx.x];
+    if (tl.nrows <= 0) return;
     const NodeDev nd = tl.nd;
     const int s = nd.s, fs = nd.s + nd.u;
     const cx<T>* Lf = L + nd.l;
     const cx<T>* Lt = LT + nd.l;
     __shared__ cx<T> vs[CC][KB];
+    __shared__ int arrived;
     T* red = reinterpret_cast<T*>(&vs[0][0]);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave / WK, wk = wave % WK;
@@ -853,6 +931,35 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
                     }
                 }
             }
+        }
+    }
+    if (redc != nullptr && tl.slot >= 0) {
+        const int rix = tl.pad_;
+        WaitVmem();
+        LdsBarrier();  // every wave's partial sums have reached the L2
+        if (tid == 0) arrived = AddAgent(redc + rix, 1);
+        LdsBarrier();
+        const SolveTile it = reds[rix];  // row block: first slot, slot count
+        if (__builtin_amdgcn_readfirstlane(arrived) == it.col1 - 1) {
+            const L2Buf<T> lP(P);
+            constexpr int kS = 16;  // slots loaded together (more: in groups)
+            for (int e = tid; e < it.nrows * k; e += NT) {
+                const int rl = e / k, r = e % k;
+                cx<T> a = czero<T>();
+                for (int j0 = 0; j0 < it.col1; j0 += kS) {
+                    // all of a group's loads in flight at once, then summed in
+                    // slot order (slots past the block add nothing)
+                    cx<T> v[kS];
+#pragma unroll
+                    for (int j = 0; j < kS; ++j)
+                        v[j] = lP[((int64_t)(it.col0 + min(j0 + j, it.col1 - 1)) * 64 + rl) * k + r];
+#pragma unroll
+                    for (int j = 0; j < kS; ++j)
+                        if (j0 + j < it.col1) a = a + v[j];
+                }
+                FinishGather<T, PH>(it.nd, gs, Xp, W, it.row0 + rl, r, k, a);
+            }
+            if (tid == 0) redc[rix] = 0;  // no slice of this launch counts any more
         }
     }
 }
@@ -1229,58 +1336,6 @@ __global__ __launch_bounds__(256) void k_mvp(const SolveTile* tiles, int start, 
 // state (counters, heads) is cleared by the copy-in or zero-set launch
 // that precedes each panel solve.
 // --------------------------------------------------------------------------
-__device__ __forceinline__ int XccId()
-{
-    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20);  // HW_REG_XCC_ID, bits 3:0
-}
-
-__device__ __forceinline__ int LoadAgent(const int32_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ int AddAgent(int32_t* p, int v)
-{
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// This wave's stores (and loads) have completed: s_waitcnt vmcnt(0).
-__device__ __forceinline__ void WaitVmem() { __builtin_amdgcn_s_waitcnt(0xF70); }
-
-// Loads of vectors that another workgroup of the XCD wrote during this
-// launch: buffer loads with the device-scope cache policy (sc1), which miss
-// the CU's L1 and read the L2, where the producer's stores are.  They are
-// ordinary loads to the compiler (scheduled and batched like any other).
-// Measured on MI355X: with sc0 loads (workgroup scope), or with an L1
-// invalidate after the wait, results went wrong whenever a vector row was
-// not a whole number of 128-byte lines (a line then holds rows of two nodes,
-// and the L1 can keep or regain a copy read before the other node was
-// written); with sc1 loads every case is exact, and so are full
-// device-scope release/acquire fences, which also write back and
-// invalidate the L2.  A buffer spans at most 4 GiB (the host checks).
-template <typename T>
-struct L2Buf {
-    __amdgpu_buffer_rsrc_t r;
-    __device__ explicit L2Buf(const void* base)
-        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000))
-    {
-    }
-    __device__ cx<T> operator[](int64_t i) const
-    {
-        constexpr int kSc1 = 16;  // cache policy: SC1 set (device scope)
-        const uint32_t off = (uint32_t)(i * (int64_t)sizeof(cx<T>));
-        cx<T> v;
-        if constexpr (sizeof(T) == 4) {
-            const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kSc1);
-            __builtin_memcpy(&v, &x, 8);
-        } else {
-            const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kSc1);
-            __builtin_memcpy(&v, &x, 16);
-        }
-        return v;
-    }
-};
-
 // The fused launch's buffers read through the L2: slot copies, work vectors,
 // panel vector, partial sums.
 template <typename T>
@@ -1750,6 +1805,7 @@ __global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
     __shared__ cx<T> Bs[2][kBK][65];
     __shared__ int32_t ix[PH == PH_F ? 1 : IX];
     const SolveTile tl = tiles[start + blockIdx.x];
+    if (tl.nrows <= 0) return;  // a hole of the XCD placement
     const NodeDev nd = tl.nd;
     const int s = nd.s, fs = nd.s + nd.u;
     const cx<T>* Lf = L + nd.l;
@@ -2325,6 +2381,24 @@ inline unsigned Blocks(int64_t n) { return (unsigned)((n + kThreads - 1) / kThre
 // ==========================================================================
 // Launch wrappers
 // ==========================================================================
+bool XcdRoundRobin(hipStream_t s)
+{
+    constexpr int kBlocks = 64;
+    int32_t* d = nullptr;
+    if (hipMalloc(&d, sizeof(int32_t) * kBlocks) != hipSuccess) return false;
+    int32_t h[kBlocks] = {};
+    bool ok = hipMemsetAsync(d, 0xff, sizeof(h), s) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_xcc_probe, dim3(kBlocks), dim3(64), 0, s, d);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s) == hipSuccess &&
+             hipStreamSynchronize(s) == hipSuccess;
+    }
+    (void)hipFree(d);
+    for (int b = 0; ok && b < kBlocks; ++b) ok = h[b] == b % 8;
+    return ok;
+}
+
 void LaunchFillZero(void* p, int64_t bytes, hipStream_t s)
 {
     if (bytes <= 0) return;
@@ -2460,6 +2534,7 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
     auto mv = [&](auto ph, const ClassPlan::SolveLevel& lv, const ClassPlan::Range& r) {
         if (r.count <= 0) return;
         constexpr int PH = decltype(ph)::value;
+        bool reduced = false;  // split row blocks already summed by the product launch
         if (prof) (void)hipEventRecord(prof->a, s);
         if (k >= 32) {
             if (r.rows == 64)
@@ -2491,14 +2566,21 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
             else
                 byRowsP(std::integral_constant<int, 16>());
         } else if (k >= 8) {
+            // split row blocks summed in the launch (slices placed per XCD;
+            // the partial sums are read through a buffer of at most 4 GiB)
+            const bool pfits = (int64_t)sizeof(cx<T>) * k * 64 * plan.maxSlots < ((int64_t)1 << 32);
+            int32_t* rc = r.xcd && c.redc && pfits ? c.redc : nullptr;
+            reduced = rc != nullptr;
             auto launch = [&](auto rt, auto kb) {
                 constexpr int RT = decltype(rt)::value, KB = decltype(kb)::value;
                 if (r.waves == 16)
                     hipLaunchKernelGGL((k_mvm<T, PH, RT, KB, 16>), dim3(r.count), dim3(1024), 0,
-                                       s, c.mv, r.start, c.structs, gs, L, LT, Xp, W, P, k);
+                                       s, c.mv, r.start, c.structs, gs, L, LT, Xp, W, P, k, rc,
+                                       c.mvred);
                 else
                     hipLaunchKernelGGL((k_mvm<T, PH, RT, KB, 4>), dim3(r.count), dim3(256), 0, s,
-                                       c.mv, r.start, c.structs, gs, L, LT, Xp, W, P, k);
+                                       c.mv, r.start, c.structs, gs, L, LT, Xp, W, P, k, rc,
+                                       c.mvred);
             };
             auto byRows = [&](auto kb) {
                 if (r.rows == 64)
@@ -2532,7 +2614,7 @@ void LaunchPanelSolve(const ClassDev& c, const ClassPlan& plan, int b, void* Xpv
             prof->bytes += eb * (lv.entries[PH] + lv.vecRows[PH] * k);
             prof->flops += 8.0 * lv.entries[PH] * k;
         }
-        if (r.rcount > 0)
+        if (r.rcount > 0 && !reduced)
             hipLaunchKernelGGL((k_mv_reduce<T, PH>), dim3(r.rcount), dim3(kThreads), 0, s,
                                c.mvred, r.rstart, gs, Xp, W, P, k);
     };

@@ -73,6 +73,8 @@ struct ClassDev {
     NodeDev* nodes = nullptr;
     SolveTile* mv = nullptr;
     SolveTile* mvred = nullptr;     // reduce items of split solve row blocks
+    int32_t* redcBuf = nullptr;     // one arrival counter per reduce item (zero between launches)
+    int32_t* redc = nullptr;        // redcBuf when split row blocks are summed in the launch
     TransItem* trans = nullptr;     // LT[:, :s] blocks
     TransItem* foldBack = nullptr;  // L_BL / diagonal blocks
     int32_t* ginv = nullptr;
@@ -105,6 +107,9 @@ inline int QueueSyncInts(int nnodes, int nqueues, int nreds) { return nnodes + n
 
 // ---- launch wrappers (kernels.hip) ----
 void LaunchFillZero(void* p, int64_t bytes, hipStream_t s);
+// True when workgroup b of a launch runs on XCD b % 8 (hardware register
+// XCC_ID), checked with a 64-workgroup probe on stream s (not capturing).
+bool XcdRoundRobin(hipStream_t s);
 // Occupancy of the persistent solve kernels, queried once (outside any
 // stream capture).
 void PrimeSolveOccupancy();

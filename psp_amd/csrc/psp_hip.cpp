@@ -11,6 +11,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <mutex>
 #include <thread>
@@ -65,6 +66,8 @@ struct psp_hip_ctx {
     // captured sweeps, keyed on (block pointer, k)
     bool useGraph = true;
     int mvpGrid = 0;  // persistent solve launches: workgroup cap (0: none)
+    bool xcdReduce = true;  // split row blocks summed in the product launch (option)
+    bool xcdOk = false;     // the device deals workgroups to its XCDs round-robin
     struct SweepGraph {
         void* B = nullptr;
         int k = 0;
@@ -213,9 +216,35 @@ std::vector<SolveTile> SolveTiles(const ClassPlan& P, const std::vector<MvTile>&
         t.col0 = m.col0;
         t.col1 = m.col1;
         t.slot = m.slot;
-        t.pad_ = 0;
+        t.pad_ = m.red;  // the slice's reduce item (into mvred), -1 if none
     }
     return out;
+}
+
+// Zeroed arrival counters, one per reduce item of the class.
+std::vector<int32_t> ReduceCounters(const ClassPlan& P)
+{
+    return std::vector<int32_t>(std::max<size_t>(1, P.mvred.size()), 0);
+}
+
+// Whether the product launches of a class sum split row blocks themselves
+// (the option, and the device's round-robin XCD dealing, checked once per
+// device).
+bool XcdDealing(psp_hip_ctx* c)
+{
+    static std::mutex mu;
+    static std::map<int, bool> known;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = known.find(dev);
+    if (it == known.end()) it = known.emplace(dev, XcdRoundRobin(c->stream)).first;
+    return it->second;
+}
+
+void SetSliceReduce(psp_hip_ctx* c, ClassDev& D)
+{
+    D.redc = c->xcdReduce && c->xcdOk ? D.redcBuf : nullptr;
 }
 
 int UploadQueues(psp_hip_ctx* c, int cl);
@@ -262,6 +291,7 @@ int EnsureDevice(psp_hip_ctx* c)
         HIPCHK(c, hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
         c->stream = c->own;
     }
+    c->xcdOk = XcdDealing(c);
     const Geometry& g = c->plan.geo;
     c->classPanels.assign(3, {});
     c->panelBatch.assign(g.numPanels, 0);
@@ -350,6 +380,7 @@ int EnsureDevice(psp_hip_ctx* c)
             (st = Upload(c, &D.tiles, P.tiles)) || (st = Upload(c, &D.inv, P.inv)) ||
             (st = Upload(c, &D.mv, SolveTiles(P, P.mv))) ||
             (st = Upload(c, &D.mvred, SolveTiles(P, P.mvred))) ||
+            (st = Upload(c, &D.redcBuf, ReduceCounters(P))) ||
             (st = Upload(c, &D.trans, P.trans)) || (st = Upload(c, &D.ginv, P.ginv)) ||
             (st = Upload(c, &D.foldBack, P.foldBack)) ||
             (st = Upload(c, &D.panelVOff, vOff)))
@@ -357,6 +388,7 @@ int EnsureDevice(psp_hip_ctx* c)
         std::vector<NodeDev> nd(D.nnodes);
         for (int t = 0; t < D.nnodes; ++t) nd[t] = MakeNodeDev(P, t);
         if ((st = Upload(c, &D.nodes, nd))) return st;
+        SetSliceReduce(c, D);
         D.ntrans = (int)P.trans.size();
         D.nfold = (int)P.foldBack.size();
         std::vector<c128> s3c = ToC128(s3a);
@@ -588,8 +620,10 @@ int RebuildSolveTiles(psp_hip_ctx* c)
         ClassDev& D = c->dev[cl];
         int st;
         if ((st = ReUpload(c, &D.mv, SolveTiles(P, P.mv))) ||
-            (st = ReUpload(c, &D.mvred, SolveTiles(P, P.mvred))) || (st = UploadQueues(c, cl)))
+            (st = ReUpload(c, &D.mvred, SolveTiles(P, P.mvred))) ||
+            (st = ReUpload(c, &D.redcBuf, ReduceCounters(P))) || (st = UploadQueues(c, cl)))
             return st;
+        SetSliceReduce(c, D);
     }
     FreeWorkspace(c);
     return PSP_HIP_OK;
@@ -1047,8 +1081,10 @@ int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
             int st;
             if ((st = ReUpload(c, &D.mv, SolveTiles(P, P.mv))) ||
                 (st = ReUpload(c, &D.mvred, SolveTiles(P, P.mvred))) ||
+                (st = ReUpload(c, &D.redcBuf, ReduceCounters(P))) ||
                 (st = EnsureWorkspace(c, k, 1)))
                 return st;
+            SetSliceReduce(c, D);
             const size_t eb_ = FrontBytes(c);
             LaunchFillZero(c->Xp, (int64_t)(eb_ * P.n * k), s);
             LaunchFillZero(c->W, (int64_t)(eb_ * P.wstore * k), s);
@@ -1078,10 +1114,10 @@ int Autotune(psp_hip_ctx* c, int k, double* before, double* after)
                 double tbest = tcur;
                 std::vector<SolveShape> cands;
                 for (int rows : {64, 32, 16})
-                    for (int split : {1, 2, 4, 8}) cands.push_back(SolveShape{rows, split, 4});
+                    for (int split : {1, 2, 4, 8, 16}) cands.push_back(SolveShape{rows, split, 4});
                 if (k >= 8 && k < 32)  // 16-wave workgroups exist on the matrix-core path
                     for (int rows : {32, 16})
-                        for (int split : {1, 2}) cands.push_back(SolveShape{rows, split, 16});
+                        for (int split : {1, 2, 4}) cands.push_back(SolveShape{rows, split, 16});
                 if (k >= 8 && k <= 16)  // persistent pipelined workgroups
                     for (int rows : {64, 32, 16})
                         for (int split : {1, 2, 4}) cands.push_back(SolveShape{rows, split, 4, 1});
@@ -1188,6 +1224,13 @@ int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
         if (c->devReady) (void)hipStreamSynchronize(c->stream);
         FreeGraphs(c);
         c->useGraph = value != 0;
+        return PSP_HIP_OK;
+    }
+    if (option == PSP_HIP_OPT_SOLVE_XCD_REDUCE) {
+        if (c->devReady) (void)hipStreamSynchronize(c->stream);
+        FreeGraphs(c);  // captured launches keep their reduce launches or counters
+        c->xcdReduce = value != 0;
+        for (auto& D : c->dev) SetSliceReduce(c, D);
         return PSP_HIP_OK;
     }
     if (option == PSP_HIP_OPT_SOLVE_PERSIST_GRID) {

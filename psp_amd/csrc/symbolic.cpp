@@ -681,6 +681,7 @@ void BuildSolveTiles(ClassPlan& C)
                 red.col0 = slots;  // first slot
                 red.col1 = nsl;    // slot count
                 red.slot = 0;
+                m.red = (int)C.mvred.size();
                 C.mvred.push_back(red);
                 for (int j = 0; j < nsl; ++j) {
                     m.col0 = lo + j * per;
@@ -693,6 +694,40 @@ void BuildSolveTiles(ClassPlan& C)
         rg.count = (int)C.mv.size() - rg.start;
         rg.rcount = (int)C.mvred.size() - rg.rstart;
         C.maxSlots = std::max(C.maxSlots, slots);
+    };
+    // XCD placement of a one-workgroup-per-tile level with split row blocks:
+    // workgroup i of a launch runs on XCD i % 8, so every slice of a row
+    // block goes to a position with the same residue mod 8 (a row block per
+    // residue, least loaded residue first); the last slice to finish can then
+    // sum the block in the launch, with hand-offs inside one XCD's L2.  The
+    // positions a residue leaves free are holes (tiles with no rows).
+    auto place = [&](ClassPlan::Range& rg) {
+        rg.xcd = 0;
+        if (rg.persist || rg.rcount <= 0) return;
+        std::vector<std::vector<MvTile>> lane(8);
+        for (int i = rg.start; i < rg.start + rg.count;) {
+            int j = i + 1;
+            if (C.mv[i].red >= 0)
+                while (j < rg.start + rg.count && C.mv[j].red == C.mv[i].red) ++j;
+            int best = 0;
+            for (int q = 1; q < 8; ++q)
+                if (lane[q].size() < lane[best].size()) best = q;
+            for (int t = i; t < j; ++t) lane[best].push_back(C.mv[t]);
+            i = j;
+        }
+        size_t len = 0;
+        for (const auto& l : lane) len = std::max(len, l.size());
+        MvTile hole = C.mv[rg.start];
+        hole.nrows = 0;
+        hole.slot = -1;
+        hole.red = -1;
+        std::vector<MvTile> placed(8 * len, hole);
+        for (int q = 0; q < 8; ++q)
+            for (size_t t = 0; t < lane[q].size(); ++t) placed[8 * t + q] = lane[q][t];
+        C.mv.erase(C.mv.begin() + rg.start, C.mv.end());
+        C.mv.insert(C.mv.end(), placed.begin(), placed.end());
+        rg.count = (int)placed.size();
+        rg.xcd = 1;
     };
     // Compulsory work of a phase: the factor entries it applies, and per
     // node its inner vector rows read once plus its output rows.
@@ -727,9 +762,11 @@ void BuildSolveTiles(ClassPlan& C)
     for (int d = 0; d <= C.maxDepth; ++d) {
         ClassPlan::SolveLevel& L = C.solveLevels[d];
         build(d, 0, L.f);  // [y; W_bot] = L g_top + [0; g_bot]
-        build(d, 1, L.b);  // x = LT [y; x_struct]
         account(L, 0, L.f);
+        place(L.f);
+        build(d, 1, L.b);  // x = LT [y; x_struct]
         account(L, 1, L.b);
+        place(L.b);
     }
 }
 

@@ -109,9 +109,10 @@ struct InvItem {
 // struct): col0 = first slot, col1 = number of slots.
 struct MvTile {
     int32_t node;
-    int32_t row0, nrows;
+    int32_t row0, nrows;  // nrows 0: a hole of the XCD placement (no work)
     int32_t col0, col1;
     int32_t slot;
+    int32_t red = -1;     // reduce item of a slice's row block (into mvred), -1 if none
 };
 
 // Fused subtree solves: one persistent launch per phase runs every level of
@@ -218,6 +219,7 @@ struct ClassPlan {
         int waves = 4;               // waves per workgroup of the matrix-core path
         int persist = 0;             // persistent pipelined workgroups
         int rstart = 0, rcount = 0;  // reduce items
+        int xcd = 0;                 // tiles placed so each row block's slices share an XCD
     };
     struct SolveLevel {
         Range f, b;

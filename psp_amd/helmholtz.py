@@ -300,7 +300,7 @@ class DistHelmholtz:
 
     def set_solve_tiling(self, min_tiles=None, max_split=None, wide_wg=None, persist=None,
                          persist_grid=None, fused=None, fused_queues=None, fused_rows=None,
-                         fused_cut=None, fused_wg=None):
+                         fused_cut=None, fused_wg=None, xcd_reduce=None):
         """Default front-solve tile shapes: workgroups wanted per launch, the
         most inner slices per row block, whether levels with few row blocks
         get 16-wave workgroups instead of slices, and whether the solves run
@@ -312,13 +312,16 @@ class DistHelmholtz:
         last workgroup runs alone), fused_rows output rows per tile and the
         cut depth fused_cut (-1: the first depth with 8 nodes); fused_wg: the
         workgroups a queue expects, which sets how far row blocks near the
-        cut are cut into inner slices (1: none)."""
+        cut are cut into inner slices (1: none).  xcd_reduce (default 1): the
+        inner slices of a row block in a one-workgroup-per-tile launch run on
+        one XCD and the last to finish sums them (0: a reduce launch)."""
         for opt, v in ((_lib.OPT_SOLVE_MIN_TILES, min_tiles), (_lib.OPT_SOLVE_MAX_SPLIT, max_split),
                        (_lib.OPT_SOLVE_WIDE_WG, wide_wg), (_lib.OPT_SOLVE_PERSIST, persist),
                        (_lib.OPT_SOLVE_PERSIST_GRID, persist_grid), (_lib.OPT_SOLVE_FUSED, fused),
                        (_lib.OPT_SOLVE_FUSED_QUEUES, fused_queues),
                        (_lib.OPT_SOLVE_FUSED_ROWS, fused_rows),
-                       (_lib.OPT_SOLVE_FUSED_CUT, fused_cut), (_lib.OPT_SOLVE_FUSED_WG, fused_wg)):
+                       (_lib.OPT_SOLVE_FUSED_CUT, fused_cut), (_lib.OPT_SOLVE_FUSED_WG, fused_wg),
+                       (_lib.OPT_SOLVE_XCD_REDUCE, xcd_reduce)):
             if v is None:
                 continue
             st = _lib.lib().psp_hip_set_option(self._ctx(), opt, int(v))

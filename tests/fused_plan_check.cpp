@@ -1,6 +1,6 @@
-// Host-side check of the fused subtree work lists (BuildQueues in
-// psp_amd/csrc/symbolic.cpp) for the grids given on the command line:
-//   fused_plan_check nx ny nz [queues rows cut wgPerQueue] ...
+// Host-side check of the solve work lists (BuildQueues and BuildSolveTiles
+// in psp_amd/csrc/symbolic.cpp) for the grid given on the command line:
+//   fused_plan_check nx ny nz [queues rows cut wgPerQueue persist maxSplit]
 // For every panel class and phase it checks that
 //  - every node at or below the cut has each of its row blocks in exactly
 //    one queue, as one tile or as slices that partition the row block's
@@ -11,8 +11,10 @@
 //    children with rows; backward: the nearest ancestor with pivots, when it
 //    is at or below the cut), each with its number of row blocks, and every
 //    row block of a dependency completes earlier in the same queue;
-//  - the per-level accounting covers the same levels.
-// Prints one line per class and exits nonzero on the first violation.
+//  - the per-level accounting covers the same levels;
+//  - the one-launch-per-level lists placed per XCD keep each split row
+//    block's slices on one XCD (checkLevels).
+// Prints lines per class and exits nonzero on the first violation.
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -143,10 +145,56 @@ static int check(const ClassPlan& C, int cls)
     return 0;
 }
 
+// The one-launch-per-level tile lists: a level whose tiles were placed per
+// XCD has a multiple of 8 tiles, holes (no rows, no slot) only there, and the
+// slices of each split row block at positions equal mod 8, each slot of the
+// block exactly once; other levels have no holes.
+static int checkLevels(const ClassPlan& C, int cls)
+{
+    int placed = 0;
+    for (int d = 0; d <= C.maxDepth; ++d)
+        for (int ph = 0; ph < 2; ++ph) {
+            const ClassPlan::Range& r = ph == 0 ? C.solveLevels[d].f : C.solveLevels[d].b;
+            if (r.start < 0 || r.start + r.count > (int)C.mv.size())
+                return fail("level range", cls, ph, d);
+            if (r.xcd && (r.persist || r.count % 8))
+                return fail("placed level shape", cls, ph, d);
+            placed += r.xcd;
+            std::map<int, int> residue;
+            std::map<int, std::set<int>> slotsSeen;
+            for (int i = r.start; i < r.start + r.count; ++i) {
+                const MvTile& m = C.mv[i];
+                if (m.nrows <= 0) {
+                    if (!r.xcd || m.slot != -1 || m.red != -1) return fail("hole", cls, ph, d, i);
+                    continue;
+                }
+                if (m.red < 0) continue;
+                if (m.red < r.rstart || m.red >= r.rstart + r.rcount)
+                    return fail("reduce item out of the level", cls, ph, d, m.red);
+                const MvTile& it = C.mvred[m.red];
+                if (m.slot < it.col0 || m.slot >= it.col0 + it.col1 || m.node != it.node ||
+                    m.row0 != it.row0)
+                    return fail("slice of another row block", cls, ph, d, i);
+                if (!slotsSeen[m.red].insert(m.slot).second) return fail("slot twice", cls, ph, d, m.slot);
+                const int res = (i - r.start) % 8;
+                if (r.xcd) {
+                    auto ins = residue.emplace(m.red, res);
+                    if (ins.first->second != res) return fail("slices on two XCDs", cls, ph, d, m.red);
+                }
+            }
+            for (int j = r.rstart; j < r.rstart + r.rcount; ++j)
+                if ((int)slotsSeen[j].size() != C.mvred[j].col1)
+                    return fail("row block slices", cls, ph, d, j);
+        }
+    std::printf("class %d: %d levels placed per XCD\n", cls, placed);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 4) {
-        std::fprintf(stderr, "usage: %s nx ny nz [queues rows cut wgPerQueue]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s nx ny nz [queues rows cut wgPerQueue persist maxSplit]\n",
+                     argv[0]);
         return 2;
     }
     psp_hip_desc d{};
@@ -169,12 +217,14 @@ int main(int argc, char** argv)
     if (argc > 5) plan.fusedRows = std::atoi(argv[5]);
     if (argc > 6) plan.fusedCut = std::atoi(argv[6]);
     if (argc > 7) plan.fusedWgPerQueue = std::atoi(argv[7]);
+    if (argc > 8) plan.persist = std::atoi(argv[8]) != 0;
+    if (argc > 9) plan.maxSplit = std::atoi(argv[9]);
     std::string msg;
     if (BuildPlan(d, plan, msg) != PSP_HIP_OK) {
         std::fprintf(stderr, "plan: %s\n", msg.c_str());
         return 2;
     }
     for (int c = 0; c < 3; ++c)
-        if (plan.present[c] && check(plan.cls[c], c)) return 1;
+        if (plan.present[c] && (check(plan.cls[c], c) || checkLevels(plan.cls[c], c))) return 1;
     return 0;
 }
