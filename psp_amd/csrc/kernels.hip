@@ -1654,13 +1654,17 @@ constexpr int kBK = 16;  // inner columns per LDS stage (solve tiles)
 // scaling included.  The transforms run when the next stage is written to
 // LDS, after this stage's matrix-core steps, so the loads of stage s + 1 are
 // in flight while stage s computes.
+// skipAbove: (lower-triangular results) a 16-row x 8-column block whose
+// rows all lie above its columns, i + skipAbove < j for tile coordinates
+// (i, j), is not multiplied; its accumulators stay zero.  1 << 30: none.
 template <typename T, int RT, bool BJFAST, int BK = kBK, int CT = 64, class LA, class FA, class LB,
           class FB>
 __device__ __forceinline__ void MmaTile(const LA& la, const FA& fa, const LB& lb, const FB& fb,
                                         int kbeg, int kend, cx<T> (*As)[BK][RT + 1],
                                         cx<T> (*Bs)[BK][CT + 1],
                                         typename Mfma<T>::acc (&acc)[TileShape<RT, CT>::RB]
-                                                                   [TileShape<RT, CT>::CB])
+                                                                   [TileShape<RT, CT>::CB],
+                                        int skipAbove = 1 << 30)
 {
     typedef TileShape<RT, CT> S;
     constexpr int kBK = BK;
@@ -1716,11 +1720,14 @@ __device__ __forceinline__ void MmaTile(const LA& la, const FA& fa, const LB& lb
             for (int a = 0; a < S::RB; ++a) av[a] = As[buf][4 * k4 + lk][16 * (wr * S::RB + a) + li];
 #pragma unroll
             for (int b = 0; b < S::CB; ++b) {
-                const cx<T> v = Bs[buf][4 * k4 + lk][8 * (wc * S::CB + b) + (li & 7)];
+                const int c0 = 8 * (wc * S::CB + b);  // the block's first column (wave-uniform)
+                if (16 * (wr * S::RB + S::RB - 1) + 15 + skipAbove < c0) continue;
+                const cx<T> v = Bs[buf][4 * k4 + lk][c0 + (li & 7)];
                 const T b1 = li < 8 ? v.r : v.i;
                 const T b2 = li < 8 ? -v.i : v.r;
 #pragma unroll
                 for (int a = 0; a < S::RB; ++a) {
+                    if (16 * (wr * S::RB + a) + 15 + skipAbove < c0) continue;
                     acc[a][b] = Mfma<T>::step(av[a].r, b1, acc[a][b]);
                     acc[a][b] = Mfma<T>::step(av[a].i, b2, acc[a][b]);
                 }
@@ -1935,10 +1942,13 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_gemm_mma(const 
         return gi < g.M ? v : czero<T>();
     };
     typename Mfma<T>::acc acc[TileShape<TS, TS>::RB][TileShape<TS, TS>::CB];
+    // lower-triangular results: blocks above the diagonal are not computed
+    // (only tiles that cross the diagonal have any)
+    const int skip = (f & G_CLOWER) ? tm - tn : 1 << 30;
     if (f & G_BTRANS) {
         auto lb = [&](int k, int j) { return B[min(tn + j, g.N - 1) + (int64_t)k * g.ldb]; };
         auto fb = [&](int, int j, cx<T> v) { return tn + j < g.N ? v : czero<T>(); };
-        MmaTile<T, TS, true, BK, TS>(la, fa, lb, fb, kbeg, kend, As, Bs, acc);
+        MmaTile<T, TS, true, BK, TS>(la, fa, lb, fb, kbeg, kend, As, Bs, acc, skip);
     } else {
         auto lb = [&](int k, int j) { return B[k + (int64_t)min(tn + j, g.N - 1) * g.ldb]; };
         auto fb = [&](int k, int j, cx<T> v) {
@@ -1946,7 +1956,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_gemm_mma(const 
             if (f & G_BUNIT) v = k > gj ? v : (k == gj ? cx<T>{T(1), T(0)} : czero<T>());
             return gj < g.N ? v : czero<T>();
         };
-        MmaTile<T, TS, false, BK, TS>(la, fa, lb, fb, kbeg, kend, As, Bs, acc);
+        MmaTile<T, TS, false, BK, TS>(la, fa, lb, fb, kbeg, kend, As, Bs, acc, skip);
     }
     auto offset = [&](int gi, int gj) {
         return (f & G_CTRANS) ? gj + (int64_t)gi * g.ldc : gi + (int64_t)gj * g.ldc;
