@@ -147,19 +147,26 @@ def bench_batch(args, dev):
             t0 = time.perf_counter()
             for w in part:
                 h = psp_amd.DistHelmholtz(w.disc, precision=prec)
+                if args.batch_tiling:
+                    h.set_solve_tiling(**{kv.split("=")[0]: int(kv.split("=")[1])
+                                          for kv in args.batch_tiling.split(",")})
                 h.Initialize(w.velocity)
                 helms.append(h)
             torch.cuda.synchronize()
             t_fac += time.perf_counter() - t0
+            if args.batch_autotune:  # tile shapes per problem, outside the timed solves
+                for w, h in zip(part, helms):
+                    h.Autotune(w.B.shape[1])
             srcs = [torch.from_numpy(w.B).to(dev) for w in part]
             for w_ in range(args.warmup):
-                psp_amd.SolveBatch(helms, [b.clone() for b in srcs], args.m, args.tol)
+                psp_amd.SolveBatch(helms, [b.clone() for b in srcs], args.m, args.tol,
+                                   threads=args.batch_threads)
             times = []
             for s_ in range(args.steps):
                 bufs = [b.clone() for b in srcs]
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
-                rp = psp_amd.SolveBatch(helms, bufs, args.m, args.tol)
+                rp = psp_amd.SolveBatch(helms, bufs, args.m, args.tol, threads=args.batch_threads)
                 torch.cuda.synchronize()
                 times.append(time.perf_counter() - t0)
                 del bufs
@@ -184,7 +191,10 @@ def bench_batch(args, dev):
             "data": "synthetic",
             "config": {"workload": "32 uneven bump problems n in {40..96}, 4-16 rhs each",
                        "rhs_total": ktot, "sizes": [w.disc.nx for w in wls],
-                       "rhs": [w.B.shape[1] for w in wls]},
+                       "rhs": [w.B.shape[1] for w in wls],
+                       "batching": ("one host thread and stream per problem" if args.batch_threads
+                                    else "lockstep GMRES, one launch per sweep step for all "
+                                         "problems")},
             "paths": res}
     print(json.dumps(line), flush=True)
 
@@ -248,6 +258,14 @@ def main():
                          "100 for thrustbelt)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-front leg")
+    ap.add_argument("--batch-tiling", default="",
+                    help="batch32: set_solve_tiling options for every problem, e.g. "
+                         "'persist=0,min_tiles=1024'")
+    ap.add_argument("--batch-autotune", action="store_true",
+                    help="batch32: Autotune(k) every problem before the timed solves")
+    ap.add_argument("--batch-threads", action="store_true",
+                    help="batch32: one host thread and stream per problem instead of the "
+                         "merged sweep launches")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default front-solve tile shapes (no tuning pass after "
                          "Initialize)")

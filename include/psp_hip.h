@@ -94,7 +94,10 @@ enum {
 
 /* Flags for psp_hip_solve. */
 enum {
-    PSP_HIP_DEVICE_POINTER = 1 /* B is a device pointer (HBM-resident)   */
+    PSP_HIP_DEVICE_POINTER = 1, /* B is a device pointer (HBM-resident)   */
+    PSP_HIP_BATCH_THREADS = 2   /* psp_hip_solve_batch: one host thread and
+                                   stream per problem instead of the merged
+                                   sweep launches */
 };
 
 /* Discretization<double> (include/psp/discretization.hpp:36-110) plus the
@@ -231,11 +234,24 @@ int psp_hip_profile_sweep(psp_hip_ctx* ctx, void* B, int k,
 int psp_hip_autotune(psp_hip_ctx* ctx, int k, double* before, double* after);
 
 /* Many independent solves at once (no reference counterpart; for callers
- * with batches of uneven problems): solve i runs psp_hip_solve(ctxs[i],
- * Bs[i], ks[i], m, relTol, maxIts, flags, &reports[i]) on its own host
- * thread and its context's stream, so the sweeps (each replayed from its
- * own hipGraph) of different problems share the GPU.  Contexts must be
- * distinct.  reports may be NULL.  Returns the first failing status, or OK. */
+ * with batches of uneven problems): solve i gives the result of
+ * psp_hip_solve(ctxs[i], Bs[i], ks[i], m, relTol, maxIts, flags,
+ * &reports[i]), each problem's GMRES (solve.hpp:241-611) stepping as it
+ * would alone.  Default: the GMRES steps of all problems in lockstep on the
+ * first context's stream, and the sweeps of all unfinished problems merged:
+ * one launch per sweep step (a panel copy, or one tree level's products or
+ * reduce) covers that step of every problem, replayed from one hipGraph;
+ * one host wait per GMRES step for all problems.  The front products of a
+ * problem then run on the one-workgroup-per-tile matrix-core kernel with
+ * its tuned tile rows, slices and waves (persistent tiles included), so a
+ * problem with k < 8 or k >= 32 sums in another order than alone (results
+ * agree to rounding; they are bitwise equal for
+ * 8 <= k < 32).  Problems with FP32 and FP64 fronts form two such batches,
+ * run in turn.  flags & PSP_HIP_BATCH_THREADS: solve i runs
+ * psp_hip_solve on its own host thread and its context's stream instead.
+ * Contexts must be distinct.  reports may be NULL (batch-wide phase times
+ * and total time in each report with the merged launches).  Returns the
+ * first failing status, or OK. */
 int psp_hip_solve_batch(psp_hip_ctx* const* ctxs, void* const* Bs, const int* ks, int n,
                         int m, double relTol, int maxIts, int flags,
                         psp_hip_solve_report* reports);

@@ -16,6 +16,7 @@ PML, DIRICHLET = 0, 1
 FRONTS_FP64, FRONTS_FP32 = 0, 1
 LDL_1D, LDL_SELINV_1D = 0, 1
 DEVICE_POINTER = 1
+BATCH_THREADS = 2
 OPT_SWEEP_GRAPH = 1
 OPT_SOLVE_MIN_TILES = 2
 OPT_SOLVE_MAX_SPLIT = 3

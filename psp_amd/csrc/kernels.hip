@@ -767,6 +767,15 @@ __global__ void k_xcc_probe(int32_t* out)
 // k_mv_reduce, bit for bit), finishes the rows and clears the counter.
 // Without redc a reduce launch follows.  Tiles with no rows are holes in that
 // placement and exit at once.
+//
+// MvmTile is the work of one workgroup on tile tl (nrows > 0); k_mvm runs it
+// on one panel solve's tiles, k_mvm_batch on the tiles of many.
+template <typename T, int PH, int RT, int KB, int NWV>
+__device__ __forceinline__ void MvmTile(const SolveTile tl, const int32_t* structs,
+                                        const Slots<T> gs, const cx<T>* L, const cx<T>* LT,
+                                        cx<T>* Xp, cx<T>* W, cx<T>* P, int k, int32_t* redc,
+                                        const SolveTile* reds);
+
 template <typename T, int PH, int RT, int KB, int NWV>
 __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveTile* tiles, int start,
                                                      const int32_t* structs,
@@ -774,6 +783,17 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
                                                      const cx<T>* LT, cx<T>* Xp, cx<T>* W,
                                                      cx<T>* P, int k, int32_t* redc,
                                                      const SolveTile* reds)
+{
+    const SolveTile tl = tiles[start + blockIdx.x];
+    if (tl.nrows <= 0) return;
+    MvmTile<T, PH, RT, KB, NWV>(tl, structs, gs, L, LT, Xp, W, P, k, redc, reds);
+}
+
+template <typename T, int PH, int RT, int KB, int NWV>
+__device__ __forceinline__ void MvmTile(const SolveTile tl, const int32_t* structs,
+                                        const Slots<T> gs, const cx<T>* L, const cx<T>* LT,
+                                        cx<T>* Xp, cx<T>* W, cx<T>* P, int k, int32_t* redc,
+                                        const SolveTile* reds)
 {
     typedef typename Mfma<T>::acc acc_t;
     constexpr int NT = 64 * NWV, WR = RT / 16, WK = NWV / WR, NB = KB / 8;
@@ -787,8 +807,6 @@ __global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm(const SolveT
     static_assert(WR * WK == NWV && CC % 4 == 0 && U >= 1, "tile shape");
     static_assert(WK == 1 || 2 * CC * KB >= WR * (WK / 2) * NB * 256,
                   "reduction scratch must fit in vs");
-    const SolveTile tl = tiles[start + blockIdx.x];
-    if (tl.nrows <= 0) return;
     const NodeDev nd = tl.nd;
     const int s = nd.s, fs = nd.s + nd.u;
     const cx<T>* Lf = L + nd.l;
@@ -1982,11 +2000,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 3) void k_gemm_mma(const 
 // Sum of the inner slices of split row blocks, in slot order, then the
 // phase's finish.  One workgroup per row block.
 template <typename T, int PH>
-__global__ __launch_bounds__(kThreads) void k_mv_reduce(const SolveTile* items, int start,
-                                                        Slots<T> gs, cx<T>* Xp,
-                                                        cx<T>* W, const cx<T>* P, int k)
+__device__ __forceinline__ void MvReduceItem(const SolveTile& it, const Slots<T>& gs, cx<T>* Xp,
+                                             cx<T>* W, const cx<T>* P, int k)
 {
-    const SolveTile it = items[start + blockIdx.x];
     const NodeDev nd = it.nd;
     for (int e = threadIdx.x; e < it.nrows * k; e += kThreads) {
         const int rl = e / k, r = e % k;
@@ -1994,6 +2010,51 @@ __global__ __launch_bounds__(kThreads) void k_mv_reduce(const SolveTile* items, 
         for (int j = 0; j < it.col1; ++j) a = a + P[((int64_t)(it.col0 + j) * 64 + rl) * k + r];
         FinishGather<T, PH>(nd, gs, Xp, W, it.row0 + rl, r, k, a);
     }
+}
+
+template <typename T, int PH>
+__global__ __launch_bounds__(kThreads) void k_mv_reduce(const SolveTile* items, int start,
+                                                        Slots<T> gs, cx<T>* Xp,
+                                                        cx<T>* W, const cx<T>* P, int k)
+{
+    MvReduceItem<T, PH>(items[start + blockIdx.x], gs, Xp, W, P, k);
+}
+
+// --------------------------------------------------------------------------
+// Batched panel solves: one launch runs the same step (a tree level's
+// products or reduce, a panel copy) of the sweeps of many independent
+// problems.  Every workgroup reads its BatchBlock: the tile or reduce item it
+// works on (tiles of the problems' own lists, unchanged), the segment (one
+// problem's panel solve: buffers, factors of that panel, k) and the problem,
+// whose flag in `active` is zero once its solve has finished (its
+// workgroups then exit at once).  The work of a workgroup is that of the
+// single-problem kernels, with the same summation order.
+// --------------------------------------------------------------------------
+template <typename T, int PH, int RT, int KB, int NWV>
+__global__ __launch_bounds__(64 * NWV) PSP_MVM_OCCUPANCY void k_mvm_batch(
+    const BatchBlock* blocks, const BatchSolveSeg* segs, const int32_t* active)
+{
+    const BatchBlock bb = blocks[blockIdx.x];
+    const SolveTile tl = *bb.item;
+    const BatchSolveSeg sg = segs[bb.seg];
+    if (!active[bb.problem] || tl.nrows <= 0) return;
+    const Slots<T> gs{(cx<T>*)sg.G, sg.half, sg.relinds};
+    MvmTile<T, PH, RT, KB, NWV>(tl, sg.structs, gs, (const cx<T>*)sg.L, (const cx<T>*)sg.LT,
+                                (cx<T>*)sg.Xp, (cx<T>*)sg.W, (cx<T>*)sg.P, sg.k, nullptr,
+                                nullptr);
+}
+
+template <typename T, int PH>
+__global__ __launch_bounds__(kThreads) void k_mv_reduce_batch(const BatchBlock* blocks,
+                                                              const BatchSolveSeg* segs,
+                                                              const int32_t* active)
+{
+    const BatchBlock bb = blocks[blockIdx.x];
+    const SolveTile it = *bb.item;
+    const BatchSolveSeg sg = segs[bb.seg];
+    if (!active[bb.problem]) return;
+    const Slots<T> gs{(cx<T>*)sg.G, sg.half, sg.relinds};
+    MvReduceItem<T, PH>(it, gs, (cx<T>*)sg.Xp, (cx<T>*)sg.W, (const cx<T>*)sg.P, sg.k);
 }
 
 // LT[:, :s] = Linv^T D^-1 per front (upper triangle; D^-1 on the diagonal),
@@ -2068,16 +2129,14 @@ __global__ __launch_bounds__(kThreads) void k_fold_back(const TransItem* items,
 // Natural-order vector <-> padded panel vector (SolvePanel's copies,
 // solve.hpp:756-778 and :825-847).
 // --------------------------------------------------------------------------
+// Panel copies, element e = (panel row q, right-hand side r) of a panel
+// vector; the kernels below run them on one panel (k_copy_in, k_copy_out,
+// k_zero_set_plane) or on panels of many problems (k_copy_batch).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_copy_in(int n, int k, const int32_t* qxy,
-                                                       const int32_t* qv, const c128* B,
-                                                       cx<T>* Xp, int nx, int ny, int nz,
-                                                       int vOffset, int padding,
-                                                       int32_t* sync, int nsync)
+__device__ __forceinline__ void CopyInElem(int64_t e, int n, int k, const int32_t* qxy,
+                                           const int32_t* qv, const c128* B, cx<T>* Xp, int nx,
+                                           int ny, int nz, int vOffset, int padding)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // the fused-launch state of the panel solve that follows starts at zero
-    for (int64_t j = e; j < nsync; j += (int64_t)gridDim.x * blockDim.x) sync[j] = 0;
     if (e >= (int64_t)n * k) return;
     const int q = (int)(e / k), r = (int)(e - (int64_t)q * k);
     const int vl = qv[q];
@@ -2092,13 +2151,41 @@ __global__ __launch_bounds__(kThreads) void k_copy_in(int n, int k, const int32_
 }
 
 template <typename T>
+__global__ __launch_bounds__(kThreads) void k_copy_in(int n, int k, const int32_t* qxy,
+                                                       const int32_t* qv, const c128* B,
+                                                       cx<T>* Xp, int nx, int ny, int nz,
+                                                       int vOffset, int padding,
+                                                       int32_t* sync, int nsync)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // the fused-launch state of the panel solve that follows starts at zero
+    for (int64_t j = e; j < nsync; j += (int64_t)gridDim.x * blockDim.x) sync[j] = 0;
+    CopyInElem<T>(e, n, k, qxy, qv, B, Xp, nx, ny, nz, vOffset, padding);
+}
+
+template <typename T>
+__device__ __forceinline__ void CopyOutElem(int64_t e, int n, int k, const int32_t* qxy,
+                                            const int32_t* qv, c128* B, const cx<T>* Xp, int nx,
+                                            int ny, int nz, int vOffset, int padding, int mode,
+                                            const StencilArgs& st, const double* vel, int v1);
+
+template <typename T>
 __global__ __launch_bounds__(kThreads) void k_copy_out(int n, int k, const int32_t* qxy,
                                                         const int32_t* qv, c128* B,
                                                         const cx<T>* Xp, int nx, int ny, int nz,
                                                         int vOffset, int padding, int mode,
                                                         StencilArgs st, const double* vel, int v1)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    CopyOutElem<T>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, k, qxy, qv, B, Xp, nx, ny,
+                   nz, vOffset, padding, mode, st, vel, v1);
+}
+
+template <typename T>
+__device__ __forceinline__ void CopyOutElem(int64_t e, int n, int k, const int32_t* qxy,
+                                            const int32_t* qv, c128* B, const cx<T>* Xp, int nx,
+                                            int ny, int nz, int vOffset, int padding, int mode,
+                                            const StencilArgs& st, const double* vel, int v1)
+{
     if (e >= (int64_t)n * k) return;
     const int q = (int)(e / k), r = (int)(e - (int64_t)q * k);
     const int vl = qv[q];
@@ -2131,6 +2218,12 @@ __global__ __launch_bounds__(kThreads) void k_copy_out(int n, int k, const int32
 // (x,y,v2+1) = natural z-1; one pass over the panel vector.  The forward
 // sweep's subdiagonal coupling is applied by k_copy_out.
 template <typename T>
+__device__ __forceinline__ void ZeroSetPlaneElem(int64_t e, int n, int k, const int32_t* qxy,
+                                                 const int32_t* qv, const StencilArgs& st,
+                                                 const double* vel, const c128* B, cx<T>* Xp,
+                                                 int v2, int vlLast);
+
+template <typename T>
 __global__ __launch_bounds__(kThreads) void k_zero_set_plane(int n, int k, const int32_t* qxy,
                                                               const int32_t* qv, StencilArgs st,
                                                               const double* vel, const c128* B,
@@ -2139,6 +2232,39 @@ __global__ __launch_bounds__(kThreads) void k_zero_set_plane(int n, int k, const
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int64_t j = e; j < nsync; j += (int64_t)gridDim.x * blockDim.x) sync[j] = 0;
+    ZeroSetPlaneElem<T>(e, n, k, qxy, qv, st, vel, B, Xp, v2, vlLast);
+}
+
+// One batched panel copy: op BATCH_COPY_IN, BATCH_COPY_OUT (mode and
+// coupling plane v1 per segment) or BATCH_ZERO_SET_PLANE, with the segment's
+// elements split into whole workgroups (BatchBlock.item unused, blk: the
+// workgroup's index inside its segment).
+template <typename T>
+__global__ __launch_bounds__(kBatchCopyThreads) void k_copy_batch(int op, const BatchBlock* blocks,
+                                                                  const BatchCopySeg* segs,
+                                                                  const int32_t* active)
+{
+    const BatchBlock bb = blocks[blockIdx.x];
+    const BatchCopySeg sg = segs[bb.seg];
+    if (!active[bb.problem]) return;
+    const int64_t e = (int64_t)bb.blk * kBatchCopyThreads + threadIdx.x;
+    if (op == BATCH_COPY_IN)
+        CopyInElem<T>(e, sg.n, sg.k, sg.qxy, sg.qv, sg.B, (cx<T>*)sg.Xp, sg.st.nx, sg.st.ny,
+                      sg.st.nz, sg.vOffset, sg.padding);
+    else if (op == BATCH_COPY_OUT)
+        CopyOutElem<T>(e, sg.n, sg.k, sg.qxy, sg.qv, sg.B, (const cx<T>*)sg.Xp, sg.st.nx,
+                       sg.st.ny, sg.st.nz, sg.vOffset, sg.padding, sg.mode, sg.st, sg.vel, sg.v1);
+    else
+        ZeroSetPlaneElem<T>(e, sg.n, sg.k, sg.qxy, sg.qv, sg.st, sg.vel, sg.B, (cx<T>*)sg.Xp,
+                            sg.v2, sg.vlLast);
+}
+
+template <typename T>
+__device__ __forceinline__ void ZeroSetPlaneElem(int64_t e, int n, int k, const int32_t* qxy,
+                                                 const int32_t* qv, const StencilArgs& st,
+                                                 const double* vel, const c128* B, cx<T>* Xp,
+                                                 int v2, int vlLast)
+{
     if (e >= (int64_t)n * k) return;
     const int q = (int)(e / k), r = (int)(e - (int64_t)q * k);
     cx<T> out = czero<T>();
@@ -2742,6 +2868,63 @@ void LaunchZeroSetPlane(const ClassDev& c, const StencilArgs& st, const double* 
                        sync ? c.nsync : 0);
 }
 
+template <typename T>
+void LaunchSolveBatch(int ph, int rows, int waves, int kb, const BatchBlock* blocks, int nblocks,
+                      const BatchSolveSeg* segs, const int32_t* active, hipStream_t s)
+{
+    if (nblocks <= 0) return;
+    auto go = [&](auto phc, auto rt) {
+        constexpr int PH = decltype(phc)::value, RT = decltype(rt)::value;
+        auto byKb = [&](auto kbc) {
+            constexpr int KB = decltype(kbc)::value;
+            if (waves == 16)
+                hipLaunchKernelGGL((k_mvm_batch<T, PH, RT, KB, 16>), dim3(nblocks), dim3(1024), 0,
+                                   s, blocks, segs, active);
+            else
+                hipLaunchKernelGGL((k_mvm_batch<T, PH, RT, KB, 4>), dim3(nblocks), dim3(256), 0, s,
+                                   blocks, segs, active);
+        };
+        if (kb == 8)
+            byKb(std::integral_constant<int, 8>());
+        else
+            byKb(std::integral_constant<int, 16>());
+    };
+    auto byRows = [&](auto phc) {
+        if (rows == 64)
+            go(phc, std::integral_constant<int, 64>());
+        else if (rows == 32)
+            go(phc, std::integral_constant<int, 32>());
+        else
+            go(phc, std::integral_constant<int, 16>());
+    };
+    if (ph == PH_F)
+        byRows(std::integral_constant<int, PH_F>());
+    else
+        byRows(std::integral_constant<int, PH_B>());
+}
+
+template <typename T>
+void LaunchReduceBatch(int ph, const BatchBlock* blocks, int nblocks, const BatchSolveSeg* segs,
+                       const int32_t* active, hipStream_t s)
+{
+    if (nblocks <= 0) return;
+    if (ph == PH_F)
+        hipLaunchKernelGGL((k_mv_reduce_batch<T, PH_F>), dim3(nblocks), dim3(kThreads), 0, s,
+                           blocks, segs, active);
+    else
+        hipLaunchKernelGGL((k_mv_reduce_batch<T, PH_B>), dim3(nblocks), dim3(kThreads), 0, s,
+                           blocks, segs, active);
+}
+
+template <typename T>
+void LaunchCopyBatch(int op, const BatchBlock* blocks, int nblocks, const BatchCopySeg* segs,
+                     const int32_t* active, hipStream_t s)
+{
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL((k_copy_batch<T>), dim3(nblocks), dim3(kBatchCopyThreads), 0, s, op,
+                       blocks, segs, active);
+}
+
 void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c128* b, c128* y,
                 int k, int mode, hipStream_t s)
 {
@@ -2837,7 +3020,13 @@ void LaunchAdjointGrad(const StencilArgs& st, const c128* u, const c128* lam, in
                                    int, int, hipStream_t, const StencilArgs*, const double*,      \
                                    int);                                                          \
     template void LaunchZeroSetPlane<T>(const ClassDev&, const StencilArgs&, const double*,       \
-                                        const c128*, void*, int, int, int, hipStream_t, int32_t*);
+                                        const c128*, void*, int, int, int, hipStream_t, int32_t*); \
+    template void LaunchSolveBatch<T>(int, int, int, int, const BatchBlock*, int,                 \
+                                      const BatchSolveSeg*, const int32_t*, hipStream_t);         \
+    template void LaunchReduceBatch<T>(int, const BatchBlock*, int, const BatchSolveSeg*,         \
+                                       const int32_t*, hipStream_t);                              \
+    template void LaunchCopyBatch<T>(int, const BatchBlock*, int, const BatchCopySeg*,            \
+                                     const int32_t*, hipStream_t);
 PSP_INST(double)
 PSP_INST(float)
 #undef PSP_INST

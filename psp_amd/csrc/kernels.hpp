@@ -104,6 +104,37 @@ struct ClassDev {
 // heads, the exit counter and one arrival counter per split row block.
 inline int QueueSyncInts(int nnodes, int nqueues, int nreds) { return nnodes + nqueues + 1 + nreds; }
 
+// ---- batched sweeps of many problems (one launch per step of all sweeps) ----
+// One workgroup of a batched launch: its tile or reduce item (products,
+// reduces), its segment and problem, and its workgroup index inside the
+// segment (copies).
+struct BatchBlock {
+    const SolveTile* item;
+    int32_t seg, problem, blk, pad_;
+};
+// One problem's panel solve in a batched product or reduce launch (front
+// precision erased: the kernels cast to their cx<T>).
+struct BatchSolveSeg {
+    const int32_t* structs;
+    void* G;                 // slot copies of the panel's class, slot 1 at G + half elements
+    int64_t half;
+    const int32_t* relinds;
+    const void *L, *LT;      // this panel's factors
+    void *Xp, *W, *P;
+    int32_t k, pad_;
+};
+enum BatchCopyOp : int { BATCH_COPY_IN = 0, BATCH_COPY_OUT = 1, BATCH_ZERO_SET_PLANE = 2 };
+constexpr int kBatchCopyThreads = 256;  // elements per workgroup of a batched copy
+// One problem's panel copy in a batched copy launch.
+struct BatchCopySeg {
+    StencilArgs st;
+    const double* vel;
+    const int32_t *qxy, *qv;
+    c128* B;
+    void* Xp;
+    int32_t n, k, vOffset, padding, mode, v1, v2, vlLast;
+};
+
 
 // ---- launch wrappers (kernels.hip) ----
 void LaunchFillZero(void* p, int64_t bytes, hipStream_t s);
@@ -167,6 +198,18 @@ template <typename T>
 void LaunchZeroSetPlane(const ClassDev& c, const StencilArgs& st, const double* vel,
                         const c128* B, void* Xp, int k, int v2, int vlLast, hipStream_t s,
                         int32_t* sync = nullptr);
+// Batched launches over nblocks BatchBlocks: products of phase ph with
+// rows-per-tile rows, waves waves per workgroup and kb (8 or 16) right-hand
+// sides per pass (k_mvm's tile work), reduce items, or panel copies (op).
+template <typename T>
+void LaunchSolveBatch(int ph, int rows, int waves, int kb, const BatchBlock* blocks, int nblocks,
+                      const BatchSolveSeg* segs, const int32_t* active, hipStream_t s);
+template <typename T>
+void LaunchReduceBatch(int ph, const BatchBlock* blocks, int nblocks, const BatchSolveSeg* segs,
+                       const int32_t* active, hipStream_t s);
+template <typename T>
+void LaunchCopyBatch(int op, const BatchBlock* blocks, int nblocks, const BatchCopySeg* segs,
+                     const int32_t* active, hipStream_t s);
 
 // y = A x (mode 0) or y = b - A x (mode 1), natural order.
 void LaunchSpmv(const StencilArgs& st, const double* vel, const c128* x, const c128* b,

@@ -1030,6 +1030,562 @@ int RunSolve(psp_hip_ctx* c, void* B, int k, int m, double relTol, int maxIts, i
     return st;
 }
 
+// ---------------------------------------------------------------------------
+// Batched solves of independent problems (psp_hip_solve_batch): the sweeps of
+// all problems advance together, one launch per step (a panel copy, or one
+// tree level's products or reduce) for every problem at that step.
+// ---------------------------------------------------------------------------
+// The merged sweep: problem i's Precondition (solve.hpp:681-734) on its GMRES
+// vector w, the steps of all problems aligned as
+//   forward panel t = 0, 1, ...: copy-in, panel solve, copy-out with the
+//   subdiagonal coupling into panel t + 1;
+//   backward step t = 0, 1, ...: panel P_i - 2 - t: zero-set-plane with the
+//   superdiagonal coupling, panel solve, additive copy-out;
+// where a panel solve is the forward phase over the tree levels from the
+// leaves (level d at step D - d, D the deepest level over all problems, so
+// the roots line up) and the backward phase from the roots.
+// A step's products go in one launch per tile shape (rows per tile, waves)
+// present at that step.  Each problem's kernels see the tiles, factors and
+// buffers of its own single-problem sweep, in the same order.
+struct BatchSweep {
+    enum Kind { PRODUCTS = 0, REDUCE = 1, COPY = 2 };
+    struct Launch {
+        int kind = PRODUCTS, ph = 0, rows = 64, waves = 4, kb = 16, op = 0;
+        int64_t blk0 = 0;
+        int nblocks = 0;
+    };
+    std::vector<Launch> launches;
+    std::vector<BatchBlock> blocks;
+    std::vector<BatchSolveSeg> solveSegs;
+    std::vector<BatchCopySeg> copySegs;
+    BatchBlock* dBlocks = nullptr;
+    BatchSolveSeg* dSolve = nullptr;
+    BatchCopySeg* dCopy = nullptr;
+    int32_t* dActive = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchSweep()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : {(void*)dBlocks, (void*)dSolve, (void*)dCopy, (void*)dActive})
+            if (p) (void)hipFree(p);
+    }
+};
+
+// Builds the merged sweep of problems cs (contexts with their workspaces for
+// ks right-hand sides) on their GMRES vectors w.
+void BuildBatchSweep(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>& ks,
+                     BatchSweep& bs)
+{
+    // stage key: sweep direction (0 forward, 1 backward), panel step,
+    // sub-step (0: first copy; 1 .. 1 + D: forward phase, level D + 1 - sub;
+    // 2 + D .. 2 + 2D: backward phase, level sub - 2 - D; 3 + 2D: copy-out),
+    // part (0 products, 1 reduce), kernel shape (rows * 10000 + waves * 100 +
+    // right-hand sides per pass: 8 for k <= 8, else 16; 0 for copies)
+    using Key = std::array<int, 5>;
+    struct Entry {
+        int problem, seg;
+        const SolveTile* items;
+        int count;  // tiles / reduce items; copies: workgroups
+    };
+    std::map<Key, std::vector<Entry>> stages;
+    int dmax = 0;
+    for (psp_hip_ctx* c : cs)
+        for (int cl = 0; cl < 3; ++cl)
+            if (c->plan.present[cl]) dmax = std::max(dmax, c->plan.cls[cl].maxDepth);
+    const int subOut = 2 * dmax + 3;
+    for (int i = 0; i < (int)cs.size(); ++i) {
+        psp_hip_ctx* c = cs[i];
+        const int k = ks[i];
+        const Geometry& g = c->plan.geo;
+        const StencilArgs st = Stencil(c);
+        const bool f32 = c->desc.precision == PSP_HIP_FRONTS_FP32;
+        auto copySeg = [&](int p, int op, int mode, int v1) {
+            const ClassDev& D = c->dev[g.cls(p)];
+            BatchCopySeg s{};
+            s.st = st;
+            s.vel = c->vel;
+            s.qxy = D.qxy;
+            s.qv = D.qv;
+            s.B = c->w;
+            s.Xp = c->Xp;
+            s.n = D.n;
+            s.k = k;
+            s.vOffset = g.vOffset(p);
+            s.padding = g.padding(p);
+            s.mode = mode;
+            s.v1 = v1;
+            s.v2 = g.panelV(p) + g.depth(p) - 1;
+            s.vlLast = g.padding(p) + g.depth(p) - 1;
+            (void)op;
+            bs.copySegs.push_back(s);
+            const int nb = (int)(((int64_t)D.n * k + kBatchCopyThreads - 1) / kBatchCopyThreads);
+            return Entry{i, (int)bs.copySegs.size() - 1, nullptr, nb};
+        };
+        auto solveSeg = [&](int p) {
+            const int cl = g.cls(p);
+            const ClassDev& D = c->dev[cl];
+            const size_t eb = f32 ? sizeof(c64) : sizeof(c128);
+            const int64_t b = c->panelBatch[p];
+            BatchSolveSeg s{};
+            s.structs = D.structs;
+            s.G = c->G[cl];
+            s.half = D.wstore * k;
+            s.relinds = D.relinds;
+            s.L = (const char*)D.L + eb * b * D.fstore;
+            s.LT = (const char*)D.LT + eb * b * D.fstore;
+            s.Xp = c->Xp;
+            s.W = c->W;
+            s.P = c->P;
+            s.k = k;
+            bs.solveSegs.push_back(s);
+            return (int)bs.solveSegs.size() - 1;
+        };
+        auto levels = [&](int dir, int t, int p) {
+            const int cl = g.cls(p);
+            const ClassPlan& plan = c->plan.cls[cl];
+            const ClassDev& D = c->dev[cl];
+            const int seg = solveSeg(p);
+            const int kb = k <= 8 ? 8 : 16;
+            for (int ph = 0; ph < 2; ++ph)
+                for (int d = 0; d <= plan.maxDepth; ++d) {
+                    const ClassPlan::SolveLevel& lv = plan.solveLevels[d];
+                    const ClassPlan::Range& r = ph == 0 ? lv.f : lv.b;
+                    const int sub = ph == 0 ? 1 + dmax - d : 2 + dmax + d;
+                    if (r.count > 0)
+                        stages[Key{dir, t, sub, 0, r.rows * 10000 + r.waves * 100 + kb}]
+                            .push_back(Entry{i, seg, D.mv + r.start, r.count});
+                    if (r.rcount > 0)
+                        stages[Key{dir, t, sub, 1, 0}].push_back(
+                            Entry{i, seg, D.mvred + r.rstart, r.rcount});
+                }
+        };
+        for (int p = 0; p < g.numPanels; ++p) {
+            stages[Key{0, p, 0, 0, 0}].push_back(copySeg(p, BATCH_COPY_IN, 0, -1));
+            levels(0, p, p);
+            const int couple = p + 1 < g.numPanels ? g.panelV(p + 1) : -1;
+            stages[Key{0, p, subOut, 0, 0}].push_back(copySeg(p, BATCH_COPY_OUT, 0, couple));
+        }
+        for (int t = 0; t + 1 < g.numPanels; ++t) {
+            const int p = g.numPanels - 2 - t;
+            stages[Key{1, t, 0, 0, 0}].push_back(copySeg(p, BATCH_ZERO_SET_PLANE, 0, -1));
+            levels(1, t, p);
+            stages[Key{1, t, subOut, 0, 0}].push_back(copySeg(p, BATCH_COPY_OUT, 1, -1));
+        }
+    }
+    for (const auto& kv : stages) {
+        const Key& key = kv.first;
+        BatchSweep::Launch L;
+        L.ph = key[2] >= 2 + dmax ? 1 : 0;  // panel-solve phase: 0 forward, 1 backward
+        const bool copy = key[2] == 0 || key[2] == subOut;
+        L.kind = copy ? BatchSweep::COPY : key[3] == 0 ? BatchSweep::PRODUCTS : BatchSweep::REDUCE;
+        if (copy)
+            L.op = key[2] == 0 ? (key[0] == 0 ? BATCH_COPY_IN : BATCH_ZERO_SET_PLANE)
+                               : BATCH_COPY_OUT;
+        L.rows = key[4] / 10000;
+        L.waves = key[4] / 100 % 100;
+        L.kb = key[4] % 100;
+        L.blk0 = (int64_t)bs.blocks.size();
+        for (const Entry& e : kv.second)
+            for (int j = 0; j < e.count; ++j) {
+                BatchBlock b{};
+                b.item = copy ? nullptr : e.items + j;
+                b.seg = e.seg;
+                b.problem = e.problem;
+                b.blk = j;
+                bs.blocks.push_back(b);
+            }
+        L.nblocks = (int)((int64_t)bs.blocks.size() - L.blk0);
+        bs.launches.push_back(L);
+    }
+}
+
+template <typename T>
+void LaunchBatchSweep(const BatchSweep& bs, hipStream_t s)
+{
+    for (const BatchSweep::Launch& L : bs.launches) {
+        const BatchBlock* b = bs.dBlocks + L.blk0;
+        if (L.kind == BatchSweep::PRODUCTS)
+            LaunchSolveBatch<T>(L.ph, L.rows, L.waves, L.kb, b, L.nblocks, bs.dSolve, bs.dActive,
+                                s);
+        else if (L.kind == BatchSweep::REDUCE)
+            LaunchReduceBatch<T>(L.ph, b, L.nblocks, bs.dSolve, bs.dActive, s);
+        else
+            LaunchCopyBatch<T>(L.op, b, L.nblocks, bs.dCopy, bs.dActive, s);
+    }
+}
+
+// Device copies of the merged sweep's work lists and its captured graph.
+int UploadBatchSweep(psp_hip_ctx* c, BatchSweep& bs, int nproblems, bool f32, hipStream_t s)
+{
+    auto up = [&](auto** dst, const auto& v) -> int {
+        const size_t bytes = sizeof(v[0]) * std::max<size_t>(v.size(), 1);
+        HIPCHK(c, hipMalloc((void**)dst, bytes));
+        if (!v.empty())
+            HIPCHK(c, hipMemcpy(*dst, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice));
+        return PSP_HIP_OK;
+    };
+    int rc;
+    if ((rc = up(&bs.dBlocks, bs.blocks))) return rc;
+    if ((rc = up(&bs.dSolve, bs.solveSegs))) return rc;
+    if ((rc = up(&bs.dCopy, bs.copySegs))) return rc;
+    if ((rc = up(&bs.dActive, std::vector<int32_t>(nproblems, 1)))) return rc;
+    HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    if (f32)
+        LaunchBatchSweep<float>(bs, s);
+    else
+        LaunchBatchSweep<double>(bs, s);
+    HIPCHK(c, hipStreamEndCapture(s, &bs.graph));
+    HIPCHK(c, hipGraphInstantiate(&bs.exec, bs.graph, nullptr, nullptr, 0));
+    return PSP_HIP_OK;
+}
+
+// Host memory the device writes and reads between waits (pinned, so copies
+// queue without blocking the host).
+struct Pinned {
+    void* p = nullptr;
+    ~Pinned()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+// InternalSolveWithGMRES (solve.hpp:241-611) for many problems in lockstep:
+// problem i runs exactly the steps of Gmres on its own context, in the same
+// order, with the sweeps of all unfinished problems merged (BatchSweep) and
+// one host wait per step for all of them.  A problem that converges, fails
+// or reaches maxIts stops; its workgroups in the merged sweep exit at once.
+int GmresBatch(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>& ks, int m,
+               double relTol, int maxIts, BatchSweep& bs, hipStream_t s,
+               std::vector<GmresOut>& outs, std::vector<int>& status)
+{
+    const int np = (int)cs.size();
+    psp_hip_ctx* c0 = cs[0];
+    struct Prob {
+        int64_t N = 0, n = 0;
+        int k = 0;
+        bool active = true;
+        std::vector<double> orig;
+        std::vector<zd> H, z, sn;
+        std::vector<double> cs;
+        double *nrm = nullptr;    // pinned: k norms
+        c128 *hc = nullptr, *yh = nullptr;  // pinned: (m + 1) k each
+    };
+    std::vector<Prob> P(np);
+    // each piece starts on a 64-byte boundary (c128 is 16-byte aligned)
+    auto round64 = [](size_t b) { return (b + 63) & ~(size_t)63; };
+    size_t pinBytes = 0;
+    for (int i = 0; i < np; ++i)
+        pinBytes += round64(sizeof(double) * ks[i]) + 2 * round64(sizeof(c128) * (m + 1) * ks[i]);
+    Pinned pin;
+    HIPCHK(c0, hipHostMalloc(&pin.p, pinBytes));
+    char* q = (char*)pin.p;
+    for (int i = 0; i < np; ++i) {
+        Prob& p = P[i];
+        p.k = ks[i];
+        p.N = (int64_t)cs[i]->desc.nx * cs[i]->desc.ny * cs[i]->desc.nz;
+        p.n = p.N * p.k;
+        p.H.assign((size_t)m * m * p.k, 0);
+        p.z.assign((size_t)(m + 1) * p.k, 0);
+        p.sn.assign((size_t)m * p.k, 0);
+        p.cs.assign((size_t)m * p.k, 0);
+        p.nrm = (double*)q;
+        q += round64(sizeof(double) * p.k);
+        p.hc = (c128*)q;
+        q += round64(sizeof(c128) * (m + 1) * p.k);
+        p.yh = (c128*)q;
+        q += round64(sizeof(c128) * (m + 1) * p.k);
+    }
+    std::vector<int32_t> act(np, 1);
+    bool actDirty = false;
+    auto stop = [&](int i, int st) {
+        P[i].active = false;
+        act[i] = 0;
+        actDirty = true;
+        status[i] = st;
+    };
+    auto fail = [&](int i, int code, const char* msg) {
+        cs[i]->err = msg;
+        stop(i, code);
+    };
+    auto anyActive = [&]() {
+        for (const Prob& p : P)
+            if (p.active) return true;
+        return false;
+    };
+    // norms of problem i's w into its pinned buffer
+    auto norms = [&](int i) -> int {
+        psp_hip_ctx* c = cs[i];
+        LaunchColNorm2(c->w, P[i].N, P[i].k, c->partD, c->normD, s);
+        HIPCHK(c, hipMemcpyAsync(P[i].nrm, c->normD, sizeof(double) * P[i].k,
+                                 hipMemcpyDeviceToHost, s));
+        return PSP_HIP_OK;
+    };
+    auto sweep = [&]() -> int {
+        if (actDirty) {
+            HIPCHK(c0, hipMemcpyAsync(bs.dActive, act.data(), sizeof(int32_t) * np,
+                                      hipMemcpyHostToDevice, s));
+            HIPCHK(c0, hipStreamSynchronize(s));  // act may change before the copy ran
+            actDirty = false;
+        }
+        HIPCHK(c0, hipGraphLaunch(bs.exec, s));
+        return PSP_HIP_OK;
+    };
+    struct Events {
+        hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    for (hipEvent_t& x : ev.e) HIPCHK(c0, hipEventCreate(&x));
+    auto span = [&](int a, int b) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev.e[a], ev.e[b]);
+        return ms * 1e-3;
+    };
+    double tP = 0, tM = 0, tA = 0;
+    int rc;
+
+    for (int i = 0; i < np; ++i) {
+        psp_hip_ctx* c = cs[i];
+        LaunchFillZero(c->x, sizeof(c128) * P[i].n, s);
+        HIPCHK(c, hipMemcpyAsync(c->w, c->b, sizeof(c128) * P[i].n, hipMemcpyDeviceToDevice, s));
+        if ((rc = norms(i))) return rc;
+    }
+    HIPCHK(c0, hipStreamSynchronize(s));
+    for (int i = 0; i < np; ++i) {
+        P[i].orig.assign(P[i].nrm, P[i].nrm + P[i].k);
+        if (HasNaN(P[i].orig)) fail(i, PSP_HIP_ERR_NAN, "Original residual norms had a NaN");
+    }
+    while (anyActive()) {
+        std::vector<bool> inCycle(np);
+        for (int i = 0; i < np; ++i) inCycle[i] = P[i].active;
+        for (int i = 0; i < np; ++i)
+            if (P[i].active)
+                HIPCHK(c0, hipMemcpyAsync(cs[i]->x0, cs[i]->x, sizeof(c128) * P[i].n,
+                                          hipMemcpyDeviceToDevice, s));
+        double t0 = Now();
+        if ((rc = sweep())) return rc;
+        for (int i = 0; i < np; ++i)
+            if (P[i].active && (rc = norms(i))) return rc;
+        HIPCHK(c0, hipStreamSynchronize(s));
+        tP += Now() - t0;
+        for (int i = 0; i < np; ++i) {
+            if (!P[i].active) continue;
+            Prob& p = P[i];
+            psp_hip_ctx* c = cs[i];
+            ++outs[i].nP;
+            std::vector<double> beta(p.nrm, p.nrm + p.k);
+            if (HasNaN(beta)) {
+                fail(i, PSP_HIP_ERR_NAN, "beta list had a NaN");
+                continue;
+            }
+            LaunchScaleCols(c->normD, c->w, c->V, p.N, p.k, s);
+            std::fill(p.z.begin(), p.z.end(), zd(0));
+            for (int r = 0; r < p.k; ++r) p.z[(size_t)r * (m + 1)] = beta[r];
+        }
+        for (int j = 0; j < m && anyActive(); ++j) {
+            HIPCHK(c0, hipEventRecord(ev.e[0], s));
+            for (int i = 0; i < np; ++i)
+                if (P[i].active)
+                    LaunchSpmv(Stencil(cs[i]), cs[i]->vel, cs[i]->V + (int64_t)j * P[i].n, nullptr,
+                               cs[i]->w, P[i].k, 0, s);
+            HIPCHK(c0, hipEventRecord(ev.e[1], s));
+            if ((rc = sweep())) return rc;
+            HIPCHK(c0, hipEventRecord(ev.e[2], s));
+            for (int i = 0; i < np; ++i) {
+                if (!P[i].active) continue;
+                psp_hip_ctx* c = cs[i];
+                const int64_t N = P[i].N, n = P[i].n;
+                const int k = P[i].k;
+                LaunchColDot(c->V, c->w, N, k, c->partC, c->Hcol, s);
+                for (int l = 0; l < j; ++l)
+                    LaunchAxpyDot(c->Hcol + (int64_t)l * k, c->V + (int64_t)l * n,
+                                  c->V + (int64_t)(l + 1) * n, c->w, N, k, c->partC,
+                                  c->Hcol + (int64_t)(l + 1) * k, s);
+                LaunchAxpyCols(c->Hcol + (int64_t)j * k, -1, c->V + (int64_t)j * n, c->w, N, k, s);
+                if ((rc = norms(i))) return rc;
+                HIPCHK(c, hipMemcpyAsync(P[i].hc, c->Hcol, sizeof(c128) * (j + 1) * k,
+                                         hipMemcpyDeviceToHost, s));
+            }
+            HIPCHK(c0, hipEventRecord(ev.e[3], s));
+            HIPCHK(c0, hipStreamSynchronize(s));
+            tM += span(0, 1);
+            tP += span(1, 2);
+            tA += span(2, 3);
+            for (int i = 0; i < np; ++i) {
+                if (!P[i].active) continue;
+                Prob& p = P[i];
+                psp_hip_ctx* c = cs[i];
+                const int k = p.k;
+                const int64_t N = p.N, n = p.n;
+                ++outs[i].nM;
+                ++outs[i].nP;
+                std::vector<double> delta(p.nrm, p.nrm + k);
+                if (HasNaN(delta)) {
+                    fail(i, PSP_HIP_ERR_NAN, "delta list had a NaN");
+                    continue;
+                }
+                bool brk = false;
+                for (int r = 0; r < k; ++r) brk = brk || delta[r] == 0.0;
+                if (brk) {
+                    fail(i, PSP_HIP_ERR_BREAKDOWN, "GMRES halted due to a (usually) lucky breakdown");
+                    continue;
+                }
+                if (j + 1 != m) LaunchScaleCols(c->normD, c->w, c->V + (int64_t)(j + 1) * n, N, k, s);
+                const char* bad = nullptr;
+                for (int r = 0; r < k && !bad; ++r) {
+                    zd* Hr = p.H.data() + (size_t)r * m * m;  // H(i,j) = Hr[i + j*m]
+                    for (int l = 0; l <= j; ++l) {
+                        const c128 a = p.hc[(size_t)l * k + r];
+                        Hr[l + (size_t)j * m] = zd(a.r, a.i);
+                    }
+                    for (int l = 0; l < j; ++l) {  // previous rotations (:442-456)
+                        const double cc = p.cs[l + (size_t)r * m];
+                        const zd sv = p.sn[l + (size_t)r * m];
+                        const zd e0 = Hr[l + (size_t)j * m], e1 = Hr[l + 1 + (size_t)j * m];
+                        Hr[l + (size_t)j * m] = cc * e0 + sv * e1;
+                        Hr[l + 1 + (size_t)j * m] = -std::conj(sv) * e0 + cc * e1;
+                    }
+                    const zd ejj = Hr[j + (size_t)j * m];
+                    if (ejj != ejj) {
+                        bad = "H(j,j) was NaN";
+                        break;
+                    }
+                    double cc;
+                    zd sv, rho;
+                    Givens(ejj, zd(delta[r]), cc, sv, rho);
+                    if (cc != cc || sv != sv || rho != rho) {
+                        bad = "Givens rotation was NaN";
+                        break;
+                    }
+                    Hr[j + (size_t)j * m] = rho;
+                    p.cs[j + (size_t)r * m] = cc;
+                    p.sn[j + (size_t)r * m] = sv;
+                    zd* zr = p.z.data() + (size_t)r * (m + 1);
+                    const zd zj = zr[j], zj1 = zr[j + 1];
+                    zr[j] = cc * zj + sv * zj1;
+                    zr[j + 1] = -std::conj(sv) * zj + cc * zj1;
+                    std::vector<zd> y(zr, zr + j + 1);  // Trsv(UPPER, NORMAL, NON_UNIT)
+                    for (int a = j; a >= 0; --a) {
+                        zd v = y[a];
+                        for (int l = a + 1; l <= j; ++l) v -= Hr[a + (size_t)l * m] * y[l];
+                        y[a] = v / Hr[a + (size_t)a * m];
+                    }
+                    for (int l = 0; l <= j; ++l)
+                        p.yh[(size_t)l * k + r] = c128{y[l].real(), y[l].imag()};
+                }
+                if (bad) {
+                    fail(i, PSP_HIP_ERR_NAN, bad);
+                    continue;
+                }
+                HIPCHK(c, hipMemcpyAsync(c->ydev, p.yh, sizeof(c128) * (j + 1) * k,
+                                         hipMemcpyHostToDevice, s));
+                LaunchUpdateX(c->x0, c->V, c->ydev, j, N, k, c->x, s);  // x := x0 + V y
+                LaunchSpmv(Stencil(c), c->vel, c->x, c->b, c->w, k, 1, s);  // w := b - A x
+                if ((rc = norms(i))) return rc;
+            }
+            HIPCHK(c0, hipStreamSynchronize(s));
+            for (int i = 0; i < np; ++i) {
+                if (!P[i].active) continue;
+                Prob& p = P[i];
+                psp_hip_ctx* c = cs[i];
+                GmresOut& o = outs[i];
+                ++o.nM;
+                std::vector<double> resid(p.nrm, p.nrm + p.k);
+                if (HasNaN(resid)) {
+                    fail(i, PSP_HIP_ERR_NAN, "resid norm list has NaN");
+                    continue;
+                }
+                double mx = 0;
+                for (int r = 0; r < p.k; ++r) mx = std::max(mx, resid[r] / p.orig[r]);
+                o.maxRel = mx;
+                ++o.it;
+                if (mx < relTol) {
+                    stop(i, PSP_HIP_OK);
+                    continue;
+                }
+                if (c->iterCb) {  // viewIterates (solve.hpp:579-585)
+                    std::vector<c128> xh((size_t)p.n);
+                    HIPCHK(c, hipMemcpy(xh.data(), c->x, sizeof(c128) * p.n, hipMemcpyDeviceToHost));
+                    c->iterCb(c->iterUser, o.it - 1, xh.data(), p.k);
+                }
+                if (maxIts > 0 && o.it >= maxIts)
+                    fail(i, PSP_HIP_ERR_NOCONV, "GMRES reached the iteration cap");
+            }
+        }
+        // a cycle counts as a restart for the problems that ran it to its end
+        // or converged in it (as Gmres counts them)
+        for (int i = 0; i < np; ++i)
+            if (inCycle[i] && (P[i].active || status[i] == PSP_HIP_OK)) ++outs[i].restarts;
+    }
+    for (GmresOut& o : outs) {
+        o.tP = tP;
+        o.tM = tM;
+        o.tA = tA;
+    }
+    return PSP_HIP_OK;
+}
+
+// psp_hip_solve_batch with the merged sweep (problems of one front
+// precision): checks, workspaces, the right-hand sides in, lockstep GMRES,
+// the solutions out.
+int SolveBatchMerged(psp_hip_ctx* const* ctxs, void* const* Bs, const int* ks, int n, int m,
+                     double relTol, int maxIts, int flags, psp_hip_solve_report* reports)
+{
+    const double t0 = Now();
+    std::vector<psp_hip_ctx*> cs(ctxs, ctxs + n);
+    std::vector<int> kv(ks, ks + n);
+    psp_hip_ctx* c0 = cs[0];
+    int rc;
+    for (int i = 0; i < n; ++i) {
+        if ((rc = CheckSolveArgs(cs[i], Bs[i], kv[i], m))) return rc;
+        if ((rc = EnsureWorkspace(cs[i], kv[i], m))) return rc;
+    }
+    for (psp_hip_ctx* c : cs) HIPCHK(c, hipStreamSynchronize(c->stream));
+    hipStream_t s = c0->stream;
+    const bool dev = flags & PSP_HIP_DEVICE_POINTER;
+    for (int i = 0; i < n; ++i) {
+        const int64_t e = (int64_t)cs[i]->desc.nx * cs[i]->desc.ny * cs[i]->desc.nz * kv[i];
+        HIPCHK(cs[i], hipMemcpyAsync(cs[i]->b, Bs[i], sizeof(c128) * e,
+                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    }
+    BatchSweep bs;
+    BuildBatchSweep(cs, kv, bs);
+    if ((rc = UploadBatchSweep(c0, bs, n, c0->desc.precision == PSP_HIP_FRONTS_FP32, s)))
+        return rc;
+    std::vector<GmresOut> outs(n);
+    std::vector<int> st(n, PSP_HIP_OK);
+    if ((rc = GmresBatch(cs, kv, m, relTol, maxIts, bs, s, outs, st))) return rc;
+    for (int i = 0; i < n; ++i) {
+        const int64_t e = (int64_t)cs[i]->desc.nx * cs[i]->desc.ny * cs[i]->desc.nz * kv[i];
+        HIPCHK(cs[i], hipMemcpyAsync(Bs[i], cs[i]->x, sizeof(c128) * e,
+                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(c0, hipStreamSynchronize(s));
+    const double t1 = Now();
+    for (int i = 0; i < n && reports; ++i) {
+        psp_hip_solve_report* rep = &reports[i];
+        const GmresOut& o = outs[i];
+        rep->status = st[i];
+        rep->iterations = o.it;
+        rep->restarts = o.restarts;
+        rep->final_max_rel = o.maxRel;
+        rep->seconds_total = t1 - t0;
+        rep->seconds_precond = o.tP;
+        rep->seconds_multiply = o.tM;
+        rep->seconds_arnoldi = o.tA;
+        rep->num_precond = o.nP;
+        rep->num_multiply = o.nM;
+    }
+    for (int i = 0; i < n; ++i)
+        if (st[i] != PSP_HIP_OK) return st[i];
+    return PSP_HIP_OK;
+}
+
 }  // namespace
 
 namespace {
@@ -1372,6 +1928,31 @@ int psp_hip_solve_batch(psp_hip_ctx* const* ctxs, void* const* Bs, const int* ks
         for (int j = 0; j < i; ++j)
             if (ctxs[j] == ctxs[i])
                 return Fail(ctxs[i], PSP_HIP_ERR_INVALID, "solve_batch: contexts must be distinct");
+    }
+    if (n == 0) return PSP_HIP_OK;
+    if (!(flags & PSP_HIP_BATCH_THREADS)) {
+        // one merged batch per front precision, in turn
+        int first = PSP_HIP_OK;
+        for (int prec : {PSP_HIP_FRONTS_FP32, PSP_HIP_FRONTS_FP64}) {
+            std::vector<psp_hip_ctx*> gc;
+            std::vector<void*> gb;
+            std::vector<int> gk, idx;
+            for (int i = 0; i < n; ++i)
+                if (ctxs[i]->desc.precision == prec) {
+                    gc.push_back(ctxs[i]);
+                    gb.push_back(Bs[i]);
+                    gk.push_back(ks[i]);
+                    idx.push_back(i);
+                }
+            if (gc.empty()) continue;
+            std::vector<psp_hip_solve_report> gr(gc.size());
+            const int rc = SolveBatchMerged(gc.data(), gb.data(), gk.data(), (int)gc.size(), m,
+                                            relTol, maxIts, flags, gr.data());
+            if (reports)
+                for (size_t j = 0; j < idx.size(); ++j) reports[idx[j]] = gr[j];
+            if (rc != PSP_HIP_OK && first == PSP_HIP_OK) first = rc;
+        }
+        return first;
     }
     std::vector<int> st(n, PSP_HIP_OK);
     std::vector<std::thread> workers;

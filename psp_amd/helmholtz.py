@@ -512,12 +512,16 @@ class DistHelmholtz:
         return _lib.lib().psp_hip_reordered_index(self._ctx(), x, y, vl, vs)
 
 
-def SolveBatch(helms, Bs, m=20, relTol=1e-4, maxIts=0, raise_on_error=True):
-    """Solves many independent problems at once: problem i is
-    helms[i].Solve(Bs[i], m, relTol, maxIts), each on its own host thread and
-    stream, so that uneven problems fill the GPU together.  The blocks are
-    all numpy arrays (host) or all torch complex128 CUDA tensors (HBM).
-    Returns the SolveReports in order."""
+def SolveBatch(helms, Bs, m=20, relTol=1e-4, maxIts=0, raise_on_error=True, threads=False):
+    """Solves many independent problems at once: problem i gets the result of
+    helms[i].Solve(Bs[i], m, relTol, maxIts).  By default the GMRES steps of
+    all problems run in lockstep and their sweeps are merged: one launch per
+    sweep step (a panel copy, a tree level's products or its reduce) covers
+    every unfinished problem, replayed from one captured graph.  Problems
+    with FP32 and FP64 fronts form two such batches.  threads=True: each
+    problem runs its own Solve on its own host thread and stream instead.
+    The blocks are all numpy arrays (host) or all torch complex128 CUDA
+    tensors (HBM).  Returns the SolveReports in order."""
     n = len(helms)
     if len(Bs) != n:
         raise LogicError("SolveBatch: one right-hand-side block per problem")
@@ -547,8 +551,8 @@ def SolveBatch(helms, Bs, m=20, relTol=1e-4, maxIts=0, raise_on_error=True):
         import torch
         torch.cuda.current_stream().synchronize()  # the blocks are ready
     reps = (SolveReport * n)()
-    st = _lib.lib().psp_hip_solve_batch(ctxs, ptrs, ks, n, m, relTol, maxIts,
-                                        _lib.DEVICE_POINTER if (n and dev[0]) else 0, reps)
+    flags = (_lib.DEVICE_POINTER if (n and dev[0]) else 0) | (_lib.BATCH_THREADS if threads else 0)
+    st = _lib.lib().psp_hip_solve_batch(ctxs, ptrs, ks, n, m, relTol, maxIts, flags, reps)
     if st != _lib.OK and raise_on_error:
         bad = next((i for i in range(n) if reps[i].status != _lib.OK), 0)
         _raise(st, helms[bad]._h, "SolveBatch")

@@ -226,9 +226,10 @@ def test_wide_block_fp32_sweep_close_to_fp64(torch_cuda, k):
     assert err2 < 1e-5, err2
 
 
-def test_solve_batch_matches_individual_solves(torch_cuda):
-    """Uneven problems solved together (one host thread and stream each)
-    give exactly the results and iteration counts of one-by-one solves."""
+def test_threaded_solve_batch_matches_individual_solves(torch_cuda):
+    """Uneven problems solved together, one host thread and stream each
+    (SolveBatch(threads=True)), give exactly the results and iteration counts
+    of one-by-one solves."""
     torch = torch_cuda
     import psp_amd
     from psp_amd import workloads
@@ -246,7 +247,7 @@ def test_solve_batch_matches_individual_solves(torch_cuda):
         blocks.append(torch.from_numpy(B).cuda())
         ref.append(x.cpu())
         its.append(r.iterations)
-    reps = psp_amd.SolveBatch(helms, blocks, 20, 1e-9)
+    reps = psp_amd.SolveBatch(helms, blocks, 20, 1e-9, threads=True)
     torch.cuda.synchronize()
     for j in range(len(probs)):
         assert reps[j].status == 0 and reps[j].final_max_rel < 1e-9
