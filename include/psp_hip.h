@@ -83,13 +83,19 @@ enum {
                                           near the cut, row blocks are cut into inner
                                           slices until a level has about that many
                                           tiles per queue; 1: no slices */
-    PSP_HIP_OPT_SOLVE_XCD_REDUCE = 12  /* 1 (default): the inner slices of a row block
+    PSP_HIP_OPT_SOLVE_XCD_REDUCE = 12, /* 1 (default): the inner slices of a row block
                                           run on one XCD and the last to finish sums
                                           them in the product launch (one-workgroup-
                                           per-tile levels, 8 <= k < 32), when the
                                           device deals workgroups to its XCDs
                                           round-robin (checked at set-up); 0: a
                                           reduce launch follows the product launch */
+    PSP_HIP_OPT_FACTOR_OVERLAP = 13    /* 1 (default): Initialize factors the leftover
+                                          and top panels on their own streams beside
+                                          the batch of full panels; 0: one class after
+                                          another on the context's stream (the Schur
+                                          timings of the report then time the GEMM
+                                          launches alone) */
 };
 
 /* Flags for psp_hip_solve. */

@@ -29,6 +29,7 @@ OPT_SOLVE_FUSED_ROWS = 9
 OPT_SOLVE_FUSED_CUT = 10
 OPT_SOLVE_FUSED_WG = 11
 OPT_SOLVE_XCD_REDUCE = 12
+OPT_FACTOR_OVERLAP = 13
 
 # Every entry point include/psp_hip.h declares.
 EXPORTS = (

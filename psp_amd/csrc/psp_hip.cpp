@@ -43,6 +43,7 @@ struct psp_hip_ctx {
     bool factored = false;
     hipStream_t stream = nullptr;
     hipStream_t own = nullptr;
+    hipStream_t side[2] = {nullptr, nullptr};  // factorization of panel classes 1 and 2
     std::vector<void*> allocs;  // plan + factor storage
     double* vel = nullptr;
     c128 *s1 = nullptr, *s2 = nullptr, *s3 = nullptr;
@@ -68,6 +69,7 @@ struct psp_hip_ctx {
     int mvpGrid = 0;  // persistent solve launches: workgroup cap (0: none)
     bool xcdReduce = true;  // split row blocks summed in the product launch (option)
     bool xcdOk = false;     // the device deals workgroups to its XCDs round-robin
+    bool factorOverlap = true;  // panel classes 1 and 2 factored on side streams (option)
     struct SweepGraph {
         void* B = nullptr;
         int k = 0;
@@ -630,25 +632,56 @@ int RebuildSolveTiles(psp_hip_ctx* c)
 }
 
 template <typename T>
+// The panel classes are independent.  Class 0 (the bottom and full inner
+// panels, factored as one batch) runs on the context's stream; the leftover
+// and top panels (classes 1 and 2, one panel each, whose launches are
+// latency-bound) run on side streams at the same time (unless the context's
+// factorOverlap option is off).  Phase times are summed over the classes.
 int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv, double* tSchur)
 {
     const StencilArgs st = Stencil(c);
-    hipStream_t s = c->stream;
-    hipEvent_t e[4];
-    for (auto& ev : e) HIPCHK(c, hipEventCreate(&ev));
-    float acc[3] = {0, 0, 0};
-    double schur = 0;
-    std::vector<hipEvent_t> se;  // around every level's Schur-complement GEMM
+    for (hipStream_t& x : c->side)
+        if (!x) HIPCHK(c, hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    const hipStream_t ss[3] = {c->stream, c->factorOverlap ? c->side[0] : c->stream,
+                               c->factorOverlap ? c->side[1] : c->stream};
+    // every event of the factorization, destroyed on any return; an event is
+    // named by its index (the list grows while events are in use)
+    struct Events {
+        std::vector<hipEvent_t> v;
+        ~Events()
+        {
+            for (hipEvent_t x : v)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    auto newEvent = [&](unsigned flags, int* idx) -> int {
+        hipEvent_t x = nullptr;
+        HIPCHK(c, hipEventCreateWithFlags(&x, flags));
+        evs.v.push_back(x);
+        *idx = (int)evs.v.size() - 1;
+        return PSP_HIP_OK;
+    };
+    int rc, fork;
+    // fork: the velocity is uploaded; per class: phase bounds e[0..3]
+    if ((rc = newEvent(hipEventDisableTiming, &fork))) return rc;
+    HIPCHK(c, hipEventRecord(evs.v[fork], c->stream));
+    int ce[3][4] = {{-1}, {-1}, {-1}};
+    std::vector<std::array<int, 2>> se;  // around every level's Schur-complement GEMM
     for (int cl = 0; cl < 3; ++cl) {
         if (!c->plan.present[cl]) continue;
         const ClassPlan& P = c->plan.cls[cl];
         const ClassDev& D = c->dev[cl];
         const int nb = c->plan.blockNb;
-        HIPCHK(c, hipEventRecord(e[0], s));
+        hipStream_t s = ss[cl];
+        for (int i = 0; i < 4; ++i)
+            if ((rc = newEvent(hipEventDefault, &ce[cl][i]))) return rc;
+        auto mark = [&](int i) { return hipEventRecord(evs.v[ce[cl][i]], s); };
+        if (cl > 0) HIPCHK(c, hipStreamWaitEvent(s, evs.v[fork], 0));
+        HIPCHK(c, mark(0));
         LaunchFillZero(D.L, (int64_t)sizeof(cx<T>) * D.npanels * D.fstore, s);
         LaunchFillZero(D.U, (int64_t)sizeof(cx<T>) * D.npanels * D.ustore, s);
         LaunchScatterOrig<T>(D, st, c->vel, c->desc.damping, s);
-        HIPCHK(c, hipEventRecord(e[1], s));
+        HIPCHK(c, mark(1));
         for (int dpt = P.maxDepth; dpt >= 0; --dpt) {
             const ClassPlan::LevelSteps& S = P.steps[dpt];
             for (int slot = 0; slot < 2; ++slot)
@@ -659,15 +692,17 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv, dou
                 LaunchGemm<T>(D, S.trail[kb].tileStart, S.trail[kb].tileCount, s);
             }
             if (S.schur.tileCount > 0) {
-                se.resize(se.size() + 2);
-                HIPCHK(c, hipEventCreate(&se[se.size() - 2]));
-                HIPCHK(c, hipEventCreate(&se[se.size() - 1]));
-                HIPCHK(c, hipEventRecord(se[se.size() - 2], s));
+                std::array<int, 2> g;
+                if ((rc = newEvent(hipEventDefault, &g[0])) ||
+                    (rc = newEvent(hipEventDefault, &g[1])))
+                    return rc;
+                HIPCHK(c, hipEventRecord(evs.v[g[0]], s));
                 LaunchGemm<T>(D, S.schur.tileStart, S.schur.tileCount, s);
-                HIPCHK(c, hipEventRecord(se[se.size() - 1], s));
+                HIPCHK(c, hipEventRecord(evs.v[g[1]], s));
+                se.push_back(g);
             }
         }
-        HIPCHK(c, hipEventRecord(e[2], s));
+        HIPCHK(c, mark(2));
         LaunchInvDiag<T>(D, nb, s);
         for (const GemmLaunch& gl : P.invGemm) LaunchGemm<T>(D, gl.tileStart, gl.tileCount, s);
         // solve layout: LT[:, :s] = Linv^T D^-1, LT[:, s:] = (-L_BL Linv)^T, then
@@ -675,24 +710,26 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv, dou
         LaunchTransposeScale<T>(D, s);
         LaunchGemm<T>(D, P.fold.tileStart, P.fold.tileCount, s);
         LaunchFoldBack<T>(D, s);
-        HIPCHK(c, hipEventRecord(e[3], s));
+        HIPCHK(c, mark(3));
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipEventSynchronize(e[3]));
-        float ms;
-        HIPCHK(c, hipEventElapsedTime(&ms, e[0], e[1]));
-        acc[0] += ms;
-        HIPCHK(c, hipEventElapsedTime(&ms, e[1], e[2]));
-        acc[1] += ms;
-        HIPCHK(c, hipEventElapsedTime(&ms, e[2], e[3]));
-        acc[2] += ms;
+        if (cl > 0) HIPCHK(c, hipStreamWaitEvent(c->stream, evs.v[ce[cl][3]], 0));  // join
     }
-    for (size_t i = 0; i + 1 < se.size(); i += 2) {
+    float acc[3] = {0, 0, 0};
+    double schur = 0;
+    for (int cl = 0; cl < 3; ++cl) {
+        if (ce[cl][0] < 0) continue;
+        HIPCHK(c, hipEventSynchronize(evs.v[ce[cl][3]]));
         float ms;
-        HIPCHK(c, hipEventElapsedTime(&ms, se[i], se[i + 1]));
+        for (int i = 0; i < 3; ++i) {
+            HIPCHK(c, hipEventElapsedTime(&ms, evs.v[ce[cl][i]], evs.v[ce[cl][i + 1]]));
+            acc[i] += ms;
+        }
+    }
+    for (const auto& g : se) {
+        float ms;
+        HIPCHK(c, hipEventElapsedTime(&ms, evs.v[g[0]], evs.v[g[1]]));
         schur += ms * 1e-3;
     }
-    for (auto& ev : se) (void)hipEventDestroy(ev);
-    for (auto& ev : e) (void)hipEventDestroy(ev);
     *tSchur = schur;
     *tFill = acc[0] * 1e-3;
     *tLdl = acc[1] * 1e-3;
@@ -1755,6 +1792,8 @@ int psp_hip_destroy(psp_hip_ctx* c)
     FreeFactorStorage(c);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->own) (void)hipStreamDestroy(c->own);
+    for (hipStream_t x : c->side)
+        if (x) (void)hipStreamDestroy(x);
     delete c;
     return PSP_HIP_OK;
 }
@@ -1780,6 +1819,10 @@ int psp_hip_set_option(psp_hip_ctx* c, int option, int value)
         if (c->devReady) (void)hipStreamSynchronize(c->stream);
         FreeGraphs(c);
         c->useGraph = value != 0;
+        return PSP_HIP_OK;
+    }
+    if (option == PSP_HIP_OPT_FACTOR_OVERLAP) {
+        c->factorOverlap = value != 0;
         return PSP_HIP_OK;
     }
     if (option == PSP_HIP_OPT_SOLVE_XCD_REDUCE) {
@@ -1881,6 +1924,8 @@ int psp_hip_factor(psp_hip_ctx* c, const double* velocity, int scheme, psp_hip_f
                                                   : FactorClasses<double>(c, &tf, &tl, &ti, &tsc);
     if (st) {
         (void)hipStreamSynchronize(c->stream);
+        for (hipStream_t x : c->side)
+            if (x) (void)hipStreamSynchronize(x);
         FreeFactorScratch(c);
         return st;
     }

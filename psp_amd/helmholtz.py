@@ -328,6 +328,15 @@ class DistHelmholtz:
             if st != _lib.OK:
                 _raise(st, self._h, "set_option")
 
+    def set_factor_overlap(self, on):
+        """on (default): Initialize factors the leftover and top panels on
+        their own streams beside the batch of full panels; off: one panel
+        class after another, so the report's Schur-complement GEMM time
+        covers those launches alone."""
+        st = _lib.lib().psp_hip_set_option(self._ctx(), _lib.OPT_FACTOR_OVERLAP, int(bool(on)))
+        if st != _lib.OK:
+            _raise(st, self._h, "set_option")
+
     # ---- Initialize ------------------------------------------------------
     def Initialize(self, velocity, panelScheme=CLIQUE_LDL_SELINV_1D):
         """Factor every panel (initialize_finalize.hpp:12-377)."""

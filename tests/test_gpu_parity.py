@@ -351,3 +351,30 @@ def test_tile_per_workgroup_solves_match_oracle(torch_cuda, k):
     h.Precondition(xd)
     torch.cuda.synchronize()
     assert _rel(xd.cpu().numpy()[l2n], o.precondition(X[l2n])) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_factor_class_overlap_gives_the_same_factors(torch_cuda, prec):
+    """The leftover and top panels factored on side streams beside the full
+    panels (default) or one class after another: the same factors, so the
+    same sweep bit for bit; both against the oracle (FP64 fronts)."""
+    torch = torch_cuda
+    disc, vel = _problem(16, 16, 33)  # panel classes 0 (bottom, inner), 1 (leftover), 2 (top)
+    X = rand_block(np.random.default_rng(240), disc.nx * disc.ny * disc.nz, 8)
+    out = []
+    for overlap in (True, False):
+        import psp_amd
+        h = psp_amd.DistHelmholtz(disc, precision=prec)
+        h.set_factor_overlap(overlap)
+        h.Initialize(vel)
+        xd = torch.from_numpy(X).cuda()
+        h.Precondition(xd)
+        torch.cuda.synchronize()
+        out.append(xd.cpu().numpy())
+    n = h.num_panels() if callable(h.num_panels) else h.num_panels
+    assert {h.panel_info(p)["cls"] for p in range(n)} == {0, 1, 2}
+    assert np.array_equal(out[0], out[1])
+    if prec == "fp64":
+        o = _oracle(disc, vel)
+        l2n = o.local_to_natural()
+        assert _rel(out[0][l2n], o.precondition(X[l2n])) < 1e-10
