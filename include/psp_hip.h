@@ -245,9 +245,11 @@ int psp_hip_autotune(psp_hip_ctx* ctx, int k, double* before, double* after);
  * &reports[i]), each problem's GMRES (solve.hpp:241-611) stepping as it
  * would alone.  Default: the GMRES steps of all problems in lockstep on the
  * first context's stream, and the sweeps of all unfinished problems merged:
- * one launch per sweep step (a panel copy, or one tree level's products or
- * reduce) covers that step of every problem, replayed from one hipGraph;
- * one host wait per GMRES step for all problems.  The front products of a
+ * the problems form two groups of about equal work, and one launch per
+ * sweep step (a panel copy, or one tree level's products or reduce) covers
+ * that step of every problem of a group; the two groups' launch chains are
+ * parallel branches of one hipGraph, replayed for every sweep; one host
+ * wait per GMRES step for all problems.  The front products of a
  * problem then run on the one-workgroup-per-tile matrix-core kernel with
  * its tuned tile rows, slices and waves (persistent tiles included), so a
  * problem with k < 8 or k >= 32 sums in another order than alone (results

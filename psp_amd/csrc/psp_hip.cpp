@@ -1087,11 +1087,13 @@ int RunSolve(psp_hip_ctx* c, void* B, int k, int m, double relTol, int maxIts, i
 struct BatchSweep {
     enum Kind { PRODUCTS = 0, REDUCE = 1, COPY = 2 };
     struct Launch {
+        int group = 0;  // problem group: its launches form one branch of the graph
         int kind = PRODUCTS, ph = 0, rows = 64, waves = 4, kb = 16, op = 0;
         int64_t blk0 = 0;
         int nblocks = 0;
     };
     std::vector<Launch> launches;
+    int groups = 1;
     std::vector<BatchBlock> blocks;
     std::vector<BatchSolveSeg> solveSegs;
     std::vector<BatchCopySeg> copySegs;
@@ -1110,10 +1112,11 @@ struct BatchSweep {
     }
 };
 
-// Builds the merged sweep of problems cs (contexts with their workspaces for
-// ks right-hand sides) on their GMRES vectors w.
+// Appends the merged sweep of problems members (of cs: contexts with their
+// workspaces for ks right-hand sides) on their GMRES vectors w, as the
+// launches of problem group `group`.
 void BuildBatchSweep(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>& ks,
-                     BatchSweep& bs)
+                     const std::vector<int>& members, int group, BatchSweep& bs)
 {
     // stage key: sweep direction (0 forward, 1 backward), panel step,
     // sub-step (0: first copy; 1 .. 1 + D: forward phase, level D + 1 - sub;
@@ -1128,11 +1131,11 @@ void BuildBatchSweep(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>
     };
     std::map<Key, std::vector<Entry>> stages;
     int dmax = 0;
-    for (psp_hip_ctx* c : cs)
+    for (int i : members)
         for (int cl = 0; cl < 3; ++cl)
-            if (c->plan.present[cl]) dmax = std::max(dmax, c->plan.cls[cl].maxDepth);
+            if (cs[i]->plan.present[cl]) dmax = std::max(dmax, cs[i]->plan.cls[cl].maxDepth);
     const int subOut = 2 * dmax + 3;
-    for (int i = 0; i < (int)cs.size(); ++i) {
+    for (int i : members) {
         psp_hip_ctx* c = cs[i];
         const int k = ks[i];
         const Geometry& g = c->plan.geo;
@@ -1214,6 +1217,7 @@ void BuildBatchSweep(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>
     for (const auto& kv : stages) {
         const Key& key = kv.first;
         BatchSweep::Launch L;
+        L.group = group;
         L.ph = key[2] >= 2 + dmax ? 1 : 0;  // panel-solve phase: 0 forward, 1 backward
         const bool copy = key[2] == 0 || key[2] == subOut;
         L.kind = copy ? BatchSweep::COPY : key[3] == 0 ? BatchSweep::PRODUCTS : BatchSweep::REDUCE;
@@ -1239,9 +1243,10 @@ void BuildBatchSweep(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>
 }
 
 template <typename T>
-void LaunchBatchSweep(const BatchSweep& bs, hipStream_t s)
+void LaunchBatchSweep(const BatchSweep& bs, const hipStream_t* gs)
 {
     for (const BatchSweep::Launch& L : bs.launches) {
+        const hipStream_t s = gs[L.group];
         const BatchBlock* b = bs.dBlocks + L.blk0;
         if (L.kind == BatchSweep::PRODUCTS)
             LaunchSolveBatch<T>(L.ph, L.rows, L.waves, L.kb, b, L.nblocks, bs.dSolve, bs.dActive,
@@ -1253,7 +1258,9 @@ void LaunchBatchSweep(const BatchSweep& bs, hipStream_t s)
     }
 }
 
-// Device copies of the merged sweep's work lists and its captured graph.
+// Device copies of the merged sweep's work lists and its captured graph:
+// the groups' launches are parallel branches, forked from and joined to
+// stream s (the other groups captured through streams of their own).
 int UploadBatchSweep(psp_hip_ctx* c, BatchSweep& bs, int nproblems, bool f32, hipStream_t s)
 {
     auto up = [&](auto** dst, const auto& v) -> int {
@@ -1268,11 +1275,36 @@ int UploadBatchSweep(psp_hip_ctx* c, BatchSweep& bs, int nproblems, bool f32, hi
     if ((rc = up(&bs.dSolve, bs.solveSegs))) return rc;
     if ((rc = up(&bs.dCopy, bs.copySegs))) return rc;
     if ((rc = up(&bs.dActive, std::vector<int32_t>(nproblems, 1)))) return rc;
+    struct Scoped {
+        std::vector<hipStream_t> streams;
+        std::vector<hipEvent_t> events;
+        ~Scoped()
+        {
+            for (hipStream_t x : streams) (void)hipStreamDestroy(x);
+            for (hipEvent_t x : events) (void)hipEventDestroy(x);
+        }
+    } sc;
+    std::vector<hipStream_t> gs(bs.groups, s);
+    for (int g = 1; g < bs.groups; ++g) {
+        HIPCHK(c, hipStreamCreateWithFlags(&gs[g], hipStreamNonBlocking));
+        sc.streams.push_back(gs[g]);
+    }
+    for (int g = 0; g < bs.groups; ++g) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        sc.events.push_back(e);
+    }
     HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    HIPCHK(c, hipEventRecord(sc.events[0], s));  // fork
+    for (int g = 1; g < bs.groups; ++g) HIPCHK(c, hipStreamWaitEvent(gs[g], sc.events[0], 0));
     if (f32)
-        LaunchBatchSweep<float>(bs, s);
+        LaunchBatchSweep<float>(bs, gs.data());
     else
-        LaunchBatchSweep<double>(bs, s);
+        LaunchBatchSweep<double>(bs, gs.data());
+    for (int g = 1; g < bs.groups; ++g) {  // join
+        HIPCHK(c, hipEventRecord(sc.events[g], gs[g]));
+        HIPCHK(c, hipStreamWaitEvent(s, sc.events[g], 0));
+    }
     HIPCHK(c, hipStreamEndCapture(s, &bs.graph));
     HIPCHK(c, hipGraphInstantiate(&bs.exec, bs.graph, nullptr, nullptr, 0));
     return PSP_HIP_OK;
@@ -1590,8 +1622,31 @@ int SolveBatchMerged(psp_hip_ctx* const* ctxs, void* const* Bs, const int* ks, i
         HIPCHK(cs[i], hipMemcpyAsync(cs[i]->b, Bs[i], sizeof(c128) * e,
                                      dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     }
+    // Two problem groups of about equal work (grid points times right-hand
+    // sides, dealt largest first), each group's merged sweep one branch of
+    // the graph: the two chains of dependent launches overlap.  Measured on
+    // 32 problems of 40^3..96^3: 1, 2, 4 and 8 groups gave 56.7, 61.2, 59.6
+    // and 57.0 RHS/s.
+    constexpr int kBatchGroups = 2;
     BatchSweep bs;
-    BuildBatchSweep(cs, kv, bs);
+    bs.groups = std::min(n, kBatchGroups);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    auto work = [&](int i) {
+        return (double)cs[i]->desc.nx * cs[i]->desc.ny * cs[i]->desc.nz * kv[i];
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    std::vector<std::vector<int>> members(bs.groups);
+    std::vector<double> load(bs.groups, 0);
+    for (int i : order) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        members[g].push_back(i);
+        load[g] += work(i);
+    }
+    for (int g = 0; g < bs.groups; ++g) {
+        std::sort(members[g].begin(), members[g].end());
+        BuildBatchSweep(cs, kv, members[g], g, bs);
+    }
     if ((rc = UploadBatchSweep(c0, bs, n, c0->desc.precision == PSP_HIP_FRONTS_FP32, s)))
         return rc;
     std::vector<GmresOut> outs(n);
