@@ -149,6 +149,15 @@ __global__ void k_fill_zero(uint4* p, int64_t n16)
         p[i] = make_uint4(0, 0, 0, 0);
 }
 
+// rows ranges of n8 8-byte words, range r at p + r * stride8
+__global__ void k_fill_zero_rows(uint2* p, int64_t n8, int64_t stride8)
+{
+    uint2* q = p + (int64_t)blockIdx.y * stride8;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += stride)
+        q[i] = make_uint2(0, 0);
+}
+
 __global__ void k_fill_zero_bytes(unsigned char* p, int64_t n)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -2533,6 +2542,15 @@ bool XcdRoundRobin(hipStream_t s)
     (void)hipFree(d);
     for (int b = 0; ok && b < kBlocks; ++b) ok = h[b] == b % 8;
     return ok;
+}
+
+void LaunchFillZeroRows(void* p, int64_t bytes, int rows, int64_t strideBytes, hipStream_t s)
+{
+    if (bytes <= 0 || rows <= 0) return;
+    const int64_t n8 = bytes / 8;
+    const unsigned g = (unsigned)std::min<int64_t>(Blocks(n8), 4096);
+    hipLaunchKernelGGL(k_fill_zero_rows, dim3(g, rows), dim3(kThreads), 0, s, (uint2*)p, n8,
+                       strideBytes / 8);
 }
 
 void LaunchFillZero(void* p, int64_t bytes, hipStream_t s)

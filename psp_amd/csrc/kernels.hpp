@@ -138,6 +138,9 @@ struct BatchCopySeg {
 
 // ---- launch wrappers (kernels.hip) ----
 void LaunchFillZero(void* p, int64_t bytes, hipStream_t s);
+// Zeroes rows ranges of `bytes` bytes, range r at p + r * strideBytes (both
+// multiples of 8).
+void LaunchFillZeroRows(void* p, int64_t bytes, int rows, int64_t strideBytes, hipStream_t s);
 // True when workgroup b of a launch runs on XCD b % 8 (hardware register
 // XCC_ID), checked with a 64-workgroup probe on stream s (not capturing).
 bool XcdRoundRobin(hipStream_t s);

@@ -679,11 +679,14 @@ int FactorClasses(psp_hip_ctx* c, double* tFill, double* tLdl, double* tInv, dou
         if (cl > 0) HIPCHK(c, hipStreamWaitEvent(s, evs.v[fork], 0));
         HIPCHK(c, mark(0));
         LaunchFillZero(D.L, (int64_t)sizeof(cx<T>) * D.npanels * D.fstore, s);
-        LaunchFillZero(D.U, (int64_t)sizeof(cx<T>) * D.npanels * D.ustore, s);
         LaunchScatterOrig<T>(D, st, c->vel, c->desc.damping, s);
         HIPCHK(c, mark(1));
         for (int dpt = P.maxDepth; dpt >= 0; --dpt) {
             const ClassPlan::LevelSteps& S = P.steps[dpt];
+            // this level's updates start at zero (its region held the updates
+            // of depth dpt + 2, read by the extend-add of depth dpt + 1)
+            LaunchFillZeroRows((cx<T>*)D.U + S.uoff, (int64_t)sizeof(cx<T>) * S.ucount, D.npanels,
+                               (int64_t)sizeof(cx<T>) * D.ustore, s);
             for (int slot = 0; slot < 2; ++slot)
                 for (const auto& r : S.extadd[slot]) LaunchExtendAdd<T>(D, r.first, r.second, s);
             for (size_t kb = 0; kb < S.ldlDiag.size(); ++kb) {

@@ -308,21 +308,39 @@ void Schedule(const Plan& plan, ClassPlan& C)
 {
     const int nb = plan.blockNb, tile = plan.tile;
     const int nn = (int)C.nodes.size();
-    int64_t lo = 0, uo = 0, wo = 0;
+    int64_t lo = 0, wo = 0;
     for (Node& n : C.nodes) {
         const int64_t fs = n.size + n.usize;
         n.loff = lo;
         lo += fs * n.size;
-        n.uoff = uo;
-        uo += (int64_t)n.usize * n.usize;
         n.woff = wo;
         wo += fs;
     }
     C.fstore = lo;
-    C.ustore = uo;
     C.wstore = wo;
-
+    // Schur-complement workspace: the updates of depth d are written while
+    // that level is factored (the children's extend-add into the rows past
+    // the pivots, then the Schur GEMM) and read by the extend-add of depth
+    // d - 1, after which they are dead.  So the levels of even depth share
+    // one region and those of odd depth another; a level's part is zeroed
+    // just before its extend-adds.
     C.steps.assign(C.maxDepth + 1, {});
+    std::vector<int64_t> lvU(C.maxDepth + 1, 0);
+    for (const Node& n : C.nodes) lvU[n.depth] += (int64_t)n.usize * n.usize;
+    int64_t region[2] = {0, 0};
+    for (int d = 0; d <= C.maxDepth; ++d) region[d & 1] = std::max(region[d & 1], lvU[d]);
+    std::vector<int64_t> uat(C.maxDepth + 1);
+    for (int d = 0; d <= C.maxDepth; ++d) {
+        uat[d] = (d & 1) ? region[0] : 0;
+        C.steps[d].uoff = uat[d];
+        C.steps[d].ucount = lvU[d];
+    }
+    for (Node& n : C.nodes) {
+        n.uoff = uat[n.depth];
+        uat[n.depth] += (int64_t)n.usize * n.usize;
+    }
+    C.ustore = region[0] + region[1];
+
     for (int d = C.maxDepth; d >= 0; --d) {
         ClassPlan::LevelSteps& S = C.steps[d];
         const std::vector<int>& lv = C.levels[d];

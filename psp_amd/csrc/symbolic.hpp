@@ -180,6 +180,7 @@ struct ClassPlan {
 
     // Factorization schedule.
     struct LevelSteps {
+        int64_t uoff = 0, ucount = 0;                // the level's updates in BUF_U
         std::vector<std::pair<int, int>> extadd[2];  // item range per child slot
         std::vector<std::pair<int, int>> ldlDiag;    // per block step: item range
         std::vector<std::pair<int, int>> ldlTrsm;    // per block step: item range
