@@ -15,6 +15,7 @@
 //  - the one-launch-per-level lists placed per XCD keep each split row
 //    block's slices on one XCD (checkLevels).
 // Prints lines per class and exits nonzero on the first violation.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -190,6 +191,51 @@ static int checkLevels(const ClassPlan& C, int cls)
     return 0;
 }
 
+// Schur-complement workspace (Schedule): the updates of each depth lie in
+// the region of that depth's parity (even: [0, R0), odd: [R0, ustore)), the
+// level's range [steps[d].uoff, + ucount) holds exactly its nodes' updates,
+// without overlap, and the extend-add items name the child's and the
+// parent's update offsets.  Two consecutive depths then never share
+// entries, so a level's updates survive until its parent level reads them.
+static int checkUpdates(const ClassPlan& C, int cls)
+{
+    const int64_t r0 = C.maxDepth >= 1 ? C.steps[1].uoff : C.ustore;
+    for (int d = 0; d <= C.maxDepth; ++d) {
+        const ClassPlan::LevelSteps& S = C.steps[d];
+        const int64_t lo = (d & 1) ? r0 : 0, hi = (d & 1) ? C.ustore : r0;
+        if (S.uoff != lo || S.uoff + S.ucount > hi) return fail("level update range", cls, 0, d);
+        std::vector<std::pair<int64_t, int64_t>> rs;
+        int64_t sum = 0;
+        for (const Node& n : C.nodes)
+            if (n.depth == d && n.usize > 0) {
+                const int64_t e = (int64_t)n.usize * n.usize;
+                rs.push_back({n.uoff, n.uoff + e});
+                sum += e;
+            }
+        std::sort(rs.begin(), rs.end());
+        for (size_t i = 0; i < rs.size(); ++i) {
+            if (rs[i].first < S.uoff || rs[i].second > S.uoff + S.ucount)
+                return fail("update outside its level", cls, 0, d, (int)i);
+            if (i && rs[i].first < rs[i - 1].second) return fail("updates overlap", cls, 0, d, (int)i);
+        }
+        if (sum != S.ucount) return fail("level update size", cls, 0, d);
+    }
+    for (const ExtAddItem& it : C.extadd) {
+        bool found = false;
+        for (const Node& p : C.nodes)
+            for (int sl = 0; sl < 2 && !found; ++sl) {
+                const int c = p.child[sl];
+                if (c >= 0 && C.nodes[c].uoff == it.uChild && p.uoff == it.uParent &&
+                    p.loff == it.lParent && C.nodes[c].usize == it.uc)
+                    found = true;
+            }
+        if (!found) return fail("extend-add offsets", cls, 0, it.col0);
+    }
+    std::printf("class %d: updates in two regions of %lld + %lld entries\n", cls, (long long)r0,
+                (long long)(C.ustore - r0));
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 4) {
@@ -225,6 +271,8 @@ int main(int argc, char** argv)
         return 2;
     }
     for (int c = 0; c < 3; ++c)
-        if (plan.present[c] && (check(plan.cls[c], c) || checkLevels(plan.cls[c], c))) return 1;
+        if (plan.present[c] && (check(plan.cls[c], c) || checkLevels(plan.cls[c], c) ||
+                                checkUpdates(plan.cls[c], c)))
+            return 1;
     return 0;
 }

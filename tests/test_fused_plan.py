@@ -6,7 +6,9 @@ in range, every dependency done earlier in the same queue, and, in the
 one-launch-per-level lists, the slices of each split row block placed at
 positions equal mod 8 (one XCD), for the benchmark grids, small and ragged
 grids, every tile height, a deeper cut, more queues than XCDs, a single
-queue, and one-workgroup-per-tile shapes with up to 16 slices."""
+queue, and one-workgroup-per-tile shapes with up to 16 slices; and the
+factorization's Schur-complement workspace (updates of even and odd depths
+in two regions, each level's updates disjoint, extend-add offsets)."""
 import os
 import subprocess
 
@@ -37,3 +39,4 @@ def test_fused_work_lists(checker, args):
     out = subprocess.run([checker] + args.split(), capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "class 0: cut" in out.stdout and "levels placed per XCD" in out.stdout
+    assert "class 0: updates in two regions" in out.stdout
