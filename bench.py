@@ -268,6 +268,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=100)
     ap.add_argument("--k", type=int, default=8)
+    ap.add_argument("--cutoff", type=int, default=12,
+                    help="nested-dissection leaf size (points of a panel's cross-section; "
+                         "the reference's default is 12)")
     ap.add_argument("--m", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--max-its", type=int, default=-1,
@@ -361,7 +364,7 @@ def main():
         factor time, the timed seconds, the reports and one solution."""
         src = Bsrc if src is None else src
         kk = src.shape[1]
-        h = psp_amd.DistHelmholtz(disc, precision=precision)
+        h = psp_amd.DistHelmholtz(disc, precision=precision, cutoff=args.cutoff)
         if not args.xcd_reduce:
             h.set_solve_tiling(xcd_reduce=0)
         if args.fused:
