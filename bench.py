@@ -68,7 +68,7 @@ def cpu_baseline(wl, out, args):
     from oracle.oracle import Oracle, num_threads
     d, vel, B = wl.disc, wl.velocity, wl.B
     k = B.shape[1]
-    o = Oracle(d.omega, d.nx, d.ny, d.nz, d.wx, d.wy, d.wz, sigma=d.sigmax)
+    o = Oracle(d.omega, d.nx, d.ny, d.nz, d.wx, d.wy, d.wz, sigma=d.sigmax, cutoff=args.cutoff)
     cores = num_threads()
     np_ = o.num_panels
     info = [o.panel_info(p) for p in range(np_)]
@@ -110,7 +110,7 @@ def cpu_full(wl, args):
     """The oracle's whole Solve (factor + GMRES) on the host cores, timed."""
     from oracle.oracle import Oracle, num_threads
     d, vel, B = wl.disc, wl.velocity, wl.B
-    o = Oracle(d.omega, d.nx, d.ny, d.nz, d.wx, d.wy, d.wz, sigma=d.sigmax)
+    o = Oracle(d.omega, d.nx, d.ny, d.nz, d.wx, d.wy, d.wz, sigma=d.sigmax, cutoff=args.cutoff)
     t0 = time.perf_counter()
     o.factor(vel)
     t_fac = time.perf_counter() - t0
