@@ -402,6 +402,11 @@ void Schedule(const Plan& plan, ClassPlan& C)
                 }
             }
             S.ldlTrsm.push_back({l1, (int)C.ldl.size() - l1});
+            // Trailing updates in panels of two blocks: after the first block
+            // only the second block's columns are updated; after the second,
+            // every later pivot column at once with both blocks (K = 2 nb),
+            // so the trailing matrix is read and written once per two blocks.
+            const bool first = (j0 / nb) % 2 == 0;
             GemmLaunch gl;
             gl.tileStart = (int)C.tiles.size();
             for (int t : lv) {
@@ -410,16 +415,17 @@ void Schedule(const Plan& plan, ClassPlan& C)
                 const int j1 = std::min(j0 + nb, n.size);
                 if (j1 >= n.size) continue;
                 const int fs = n.size + n.usize;
+                const int p0 = first ? j0 : j0 - nb;  // the panel's first column
                 GemmDesc g{};
-                g.a = n.loff + j1 + (int64_t)j0 * fs;
+                g.a = n.loff + j1 + (int64_t)p0 * fs;
                 g.b = g.a;
                 g.c = n.loff + j1 + (int64_t)j1 * fs;
-                g.d = n.loff + j0 + (int64_t)j0 * fs;
+                g.d = n.loff + p0 + (int64_t)p0 * fs;
                 g.lda = g.ldb = g.ldc = fs;
                 g.dinc = fs + 1;
                 g.M = fs - j1;
-                g.N = n.size - j1;
-                g.K = j1 - j0;
+                g.N = first ? std::min(j1 + nb, n.size) - j1 : n.size - j1;
+                g.K = j1 - p0;
                 g.abuf = g.bbuf = g.cbuf = g.dbuf = BUF_L;
                 g.flags = G_BTRANS | G_HASD | G_NEG | G_CLOWER;
                 AddGemmTiles(C, g, tile);
