@@ -113,3 +113,24 @@ def test_merged_and_threaded_batches_agree(torch_cuda):
         assert rm[j].status == 0 and rt[j].status == 0
         assert rm[j].iterations == rt[j].iterations, j
         assert torch.equal(xm[j], xt[j]), j
+
+
+def test_merged_batch_host_blocks_and_single_problem(torch_cuda):
+    """Host (numpy) right-hand-side blocks, and a batch of one problem, give
+    the bits of one-by-one solves."""
+    torch = torch_cuda
+    import psp_amd
+    helms, Bs = [], []
+    for j, (nx, ny, nz, k) in enumerate(PROBS[:3]):
+        disc, vel = _problem(nx, ny, nz, cfg_seed=230 + j)
+        helms.append(_helm(disc, vel, "fp32"))
+        Bs.append(_sources(nx, ny, nz, k, 235 + j))
+    ref = [_alone(torch, h, B, 1e-8) for h, B in zip(helms, Bs)]
+    host = [B.copy() for B in Bs]
+    reps = psp_amd.SolveBatch(helms, host, 20, 1e-8)
+    for j, r in enumerate(reps):
+        assert r.status == 0 and r.iterations == ref[j][1].iterations, j
+        assert np.array_equal(host[j], ref[j][0].numpy()), j
+    xs, reps1 = _batch(torch, helms[:1], Bs[:1], 1e-8)
+    assert reps1[0].iterations == ref[0][1].iterations
+    assert torch.equal(xs[0], ref[0][0])
