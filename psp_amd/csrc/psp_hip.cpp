@@ -866,6 +866,49 @@ void Givens(zd f, zd g, double& c, zd& s, zd& r)
     r = ph * dd;
 }
 
+// Step j of right-hand side r's least-squares update (solve.hpp:427-470):
+// column j of its Hessenberg matrix from the Arnoldi coefficients hc
+// (element (i, r) at hc[i k + r]), the earlier rotations applied, a new
+// rotation against delta, the rotated right-hand side z, and
+// y = H(0:j, 0:j)^-1 z(0:j) into yh[i k + r].  Hr (H(i, j) = Hr[i + j m]),
+// csr, snr and zr are this right-hand side's own arrays.  Returns nullptr,
+// or the reason the step gave a NaN.
+const char* LeastSquaresStep(int m, int k, int j, int r, const c128* hc, double delta, zd* Hr,
+                             double* csr, zd* snr, zd* zr, c128* yh)
+{
+    for (int i = 0; i <= j; ++i) {
+        const c128 a = hc[(size_t)i * k + r];
+        Hr[i + (size_t)j * m] = zd(a.r, a.i);
+    }
+    for (int i = 0; i < j; ++i) {  // previous rotations (:442-456)
+        const double cc = csr[i];
+        const zd sv = snr[i];
+        const zd e0 = Hr[i + (size_t)j * m], e1 = Hr[i + 1 + (size_t)j * m];
+        Hr[i + (size_t)j * m] = cc * e0 + sv * e1;
+        Hr[i + 1 + (size_t)j * m] = -std::conj(sv) * e0 + cc * e1;
+    }
+    const zd ejj = Hr[j + (size_t)j * m];
+    if (ejj != ejj) return "H(j,j) was NaN";
+    double cc;
+    zd sv, rho;
+    Givens(ejj, zd(delta), cc, sv, rho);
+    if (cc != cc || sv != sv || rho != rho) return "Givens rotation was NaN";
+    Hr[j + (size_t)j * m] = rho;
+    csr[j] = cc;
+    snr[j] = sv;
+    const zd zj = zr[j], zj1 = zr[j + 1];
+    zr[j] = cc * zj + sv * zj1;
+    zr[j + 1] = -std::conj(sv) * zj + cc * zj1;
+    std::vector<zd> y(zr, zr + j + 1);  // Trsv(UPPER, NORMAL, NON_UNIT)
+    for (int i = j; i >= 0; --i) {
+        zd a = y[i];
+        for (int l = i + 1; l <= j; ++l) a -= Hr[i + (size_t)l * m] * y[l];
+        y[i] = a / Hr[i + (size_t)i * m];
+    }
+    for (int i = 0; i <= j; ++i) yh[(size_t)i * k + r] = c128{y[i].real(), y[i].imag()};
+    return nullptr;
+}
+
 struct GmresOut {
     int it = 0, restarts = 0;
     double maxRel = 0;
@@ -955,41 +998,12 @@ int Gmres(psp_hip_ctx* c, int k, int m, double relTol, int maxIts, int steps, do
                     return Fail(c, PSP_HIP_ERR_BREAKDOWN,
                                 "GMRES halted due to a (usually) lucky breakdown");
             if (j + 1 != m) LaunchScaleCols(c->normD, c->w, c->V + (int64_t)(j + 1) * n, N, k, s);
-            for (int r = 0; r < k; ++r) {
-                zd* Hr = H.data() + (size_t)r * m * m;  // H(i,j) = Hr[i + j*m]
-                for (int i = 0; i <= j; ++i) {
-                    const c128 a = hc[(size_t)i * k + r];
-                    Hr[i + (size_t)j * m] = zd(a.r, a.i);
-                }
-                for (int i = 0; i < j; ++i) {  // previous rotations (:442-456)
-                    const double cc = cs[i + (size_t)r * m];
-                    const zd sv = sn[i + (size_t)r * m];
-                    const zd e0 = Hr[i + (size_t)j * m], e1 = Hr[i + 1 + (size_t)j * m];
-                    Hr[i + (size_t)j * m] = cc * e0 + sv * e1;
-                    Hr[i + 1 + (size_t)j * m] = -std::conj(sv) * e0 + cc * e1;
-                }
-                const zd ejj = Hr[j + (size_t)j * m];
-                if (ejj != ejj) return Fail(c, PSP_HIP_ERR_NAN, "H(j,j) was NaN");
-                double cc;
-                zd sv, rho;
-                Givens(ejj, zd(delta[r]), cc, sv, rho);
-                if (cc != cc || sv != sv || rho != rho)
-                    return Fail(c, PSP_HIP_ERR_NAN, "Givens rotation was NaN");
-                Hr[j + (size_t)j * m] = rho;
-                cs[j + (size_t)r * m] = cc;
-                sn[j + (size_t)r * m] = sv;
-                zd* zr = z.data() + (size_t)r * (m + 1);
-                const zd zj = zr[j], zj1 = zr[j + 1];
-                zr[j] = cc * zj + sv * zj1;
-                zr[j + 1] = -std::conj(sv) * zj + cc * zj1;
-                std::vector<zd> y(zr, zr + j + 1);  // Trsv(UPPER, NORMAL, NON_UNIT)
-                for (int i = j; i >= 0; --i) {
-                    zd a = y[i];
-                    for (int l = i + 1; l <= j; ++l) a -= Hr[i + (size_t)l * m] * y[l];
-                    y[i] = a / Hr[i + (size_t)i * m];
-                }
-                for (int i = 0; i <= j; ++i) yh[(size_t)i * k + r] = c128{y[i].real(), y[i].imag()};
-            }
+            for (int r = 0; r < k; ++r)
+                if (const char* bad = LeastSquaresStep(
+                        m, k, j, r, hc.data(), delta[r], H.data() + (size_t)r * m * m,
+                        cs.data() + (size_t)r * m, sn.data() + (size_t)r * m,
+                        z.data() + (size_t)r * (m + 1), yh.data()))
+                    return Fail(c, PSP_HIP_ERR_NAN, bad);
             HIPCHK(c, hipMemcpyAsync(c->ydev, yh.data(), sizeof(c128) * (j + 1) * k,
                                      hipMemcpyHostToDevice, s));
             LaunchUpdateX(c->x0, c->V, c->ydev, j, N, k, c->x, s);  // x := x0 + V y
@@ -1509,47 +1523,10 @@ int GmresBatch(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>& ks, 
                 }
                 if (j + 1 != m) LaunchScaleCols(c->normD, c->w, c->V + (int64_t)(j + 1) * n, N, k, s);
                 const char* bad = nullptr;
-                for (int r = 0; r < k && !bad; ++r) {
-                    zd* Hr = p.H.data() + (size_t)r * m * m;  // H(i,j) = Hr[i + j*m]
-                    for (int l = 0; l <= j; ++l) {
-                        const c128 a = p.hc[(size_t)l * k + r];
-                        Hr[l + (size_t)j * m] = zd(a.r, a.i);
-                    }
-                    for (int l = 0; l < j; ++l) {  // previous rotations (:442-456)
-                        const double cc = p.cs[l + (size_t)r * m];
-                        const zd sv = p.sn[l + (size_t)r * m];
-                        const zd e0 = Hr[l + (size_t)j * m], e1 = Hr[l + 1 + (size_t)j * m];
-                        Hr[l + (size_t)j * m] = cc * e0 + sv * e1;
-                        Hr[l + 1 + (size_t)j * m] = -std::conj(sv) * e0 + cc * e1;
-                    }
-                    const zd ejj = Hr[j + (size_t)j * m];
-                    if (ejj != ejj) {
-                        bad = "H(j,j) was NaN";
-                        break;
-                    }
-                    double cc;
-                    zd sv, rho;
-                    Givens(ejj, zd(delta[r]), cc, sv, rho);
-                    if (cc != cc || sv != sv || rho != rho) {
-                        bad = "Givens rotation was NaN";
-                        break;
-                    }
-                    Hr[j + (size_t)j * m] = rho;
-                    p.cs[j + (size_t)r * m] = cc;
-                    p.sn[j + (size_t)r * m] = sv;
-                    zd* zr = p.z.data() + (size_t)r * (m + 1);
-                    const zd zj = zr[j], zj1 = zr[j + 1];
-                    zr[j] = cc * zj + sv * zj1;
-                    zr[j + 1] = -std::conj(sv) * zj + cc * zj1;
-                    std::vector<zd> y(zr, zr + j + 1);  // Trsv(UPPER, NORMAL, NON_UNIT)
-                    for (int a = j; a >= 0; --a) {
-                        zd v = y[a];
-                        for (int l = a + 1; l <= j; ++l) v -= Hr[a + (size_t)l * m] * y[l];
-                        y[a] = v / Hr[a + (size_t)a * m];
-                    }
-                    for (int l = 0; l <= j; ++l)
-                        p.yh[(size_t)l * k + r] = c128{y[l].real(), y[l].imag()};
-                }
+                for (int r = 0; r < k && !bad; ++r)
+                    bad = LeastSquaresStep(m, k, j, r, p.hc, delta[r], p.H.data() + (size_t)r * m * m,
+                                           p.cs.data() + (size_t)r * m, p.sn.data() + (size_t)r * m,
+                                           p.z.data() + (size_t)r * (m + 1), p.yh);
                 if (bad) {
                     fail(i, PSP_HIP_ERR_NAN, bad);
                     continue;
