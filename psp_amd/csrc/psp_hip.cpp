@@ -1158,7 +1158,8 @@ void BuildBatchSweep(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>
         const Geometry& g = c->plan.geo;
         const StencilArgs st = Stencil(c);
         const bool f32 = c->desc.precision == PSP_HIP_FRONTS_FP32;
-        auto copySeg = [&](int p, int op, int mode, int v1) {
+        // the copy launch's op comes from its stage (first copy or copy-out)
+        auto copySeg = [&](int p, int mode, int v1) {
             const ClassDev& D = c->dev[g.cls(p)];
             BatchCopySeg s{};
             s.st = st;
@@ -1175,7 +1176,6 @@ void BuildBatchSweep(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>
             s.v1 = v1;
             s.v2 = g.panelV(p) + g.depth(p) - 1;
             s.vlLast = g.padding(p) + g.depth(p) - 1;
-            (void)op;
             bs.copySegs.push_back(s);
             const int nb = (int)(((int64_t)D.n * k + kBatchCopyThreads - 1) / kBatchCopyThreads);
             return Entry{i, (int)bs.copySegs.size() - 1, nullptr, nb};
@@ -1219,16 +1219,16 @@ void BuildBatchSweep(const std::vector<psp_hip_ctx*>& cs, const std::vector<int>
                 }
         };
         for (int p = 0; p < g.numPanels; ++p) {
-            stages[Key{0, p, 0, 0, 0}].push_back(copySeg(p, BATCH_COPY_IN, 0, -1));
+            stages[Key{0, p, 0, 0, 0}].push_back(copySeg(p, 0, -1));
             levels(0, p, p);
             const int couple = p + 1 < g.numPanels ? g.panelV(p + 1) : -1;
-            stages[Key{0, p, subOut, 0, 0}].push_back(copySeg(p, BATCH_COPY_OUT, 0, couple));
+            stages[Key{0, p, subOut, 0, 0}].push_back(copySeg(p, 0, couple));
         }
         for (int t = 0; t + 1 < g.numPanels; ++t) {
             const int p = g.numPanels - 2 - t;
-            stages[Key{1, t, 0, 0, 0}].push_back(copySeg(p, BATCH_ZERO_SET_PLANE, 0, -1));
+            stages[Key{1, t, 0, 0, 0}].push_back(copySeg(p, 0, -1));
             levels(1, t, p);
-            stages[Key{1, t, subOut, 0, 0}].push_back(copySeg(p, BATCH_COPY_OUT, 1, -1));
+            stages[Key{1, t, subOut, 0, 0}].push_back(copySeg(p, 1, -1));
         }
     }
     for (const auto& kv : stages) {
