@@ -1825,10 +1825,24 @@ __device__ __forceinline__ void MmaForEach(const typename Mfma<T>::acc (&acc)[Ti
             }
 }
 
+// Waves per SIMD the wide-block kernel is compiled for (its register budget:
+// 512 / waves), per precision, phase and tile rows; 1 leaves the compiler
+// free.  Each value is the most that compiles without spills.  Unbounded,
+// FP32 64-row forward tiles took 293 registers (one workgroup per CU) and
+// 16-row ones 140 (three).
+template <typename T, int PH, int RT>
+constexpr int MmwWavesPerEu()
+{
+    if (sizeof(T) == 4) return RT == 16 ? 4 : 2;
+    if (RT == 16) return PH == PH_F ? 3 : 1;
+    return RT == 64 && PH == PH_B ? 2 : 1;
+}
+
 // Wide-block front solves (k >= 32): one workgroup per solve tile and pass
 // of 64 right-hand sides, through the tile engine.
 template <typename T, int PH, int RT>
-__global__ __launch_bounds__(256) void k_mmw(const SolveTile* tiles, int start,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MmwWavesPerEu<T, PH, RT>())))
+void k_mmw(const SolveTile* tiles, int start,
                                               const int32_t* structs, Slots<T> gs,
                                               const cx<T>* L, const cx<T>* LT, cx<T>* Xp,
                                               cx<T>* W, cx<T>* P, int k)
